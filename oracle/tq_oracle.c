@@ -275,7 +275,10 @@ void tqo_rotated_coords_batch(const tqo_space *s, const float *X, size_t n, floa
 void tqo_prepare_query(const tqo_space *s, const float *q, float *q_rot, float *s_q,
                        float *q_norm_sq, float *q_norm) {
   const size_t D = s->dim, n_in = s->input_dim;
-  float nsq = tqo_sumsq(q, n_in);
+  /* q_norm_sq accumulates in the returned struct (:443-445), which may alias
+   * the query, so the pinned build keeps this loop scalar: one FMA chain. */
+  float nsq = 0.0f;
+  for (size_t i = 0; i < n_in; ++i) nsq = fmaf(q[i], q[i], nsq);
   float qn = sqrtf(nsq);
   float q_inv = (qn > 1e-10f) ? (1.0f / qn) : 0.0f;
   memset(q_rot, 0, D * sizeof(float));

@@ -1,0 +1,57 @@
+// tq_common.h -- device-side view of a TurboQuant space and launch helpers
+// shared by the gfx950 kernels. Internal to libtq_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tq {
+
+// The space as the kernels see it (passed by value as a kernel argument).
+// Mirrors the constant members of TurboQuantSpace
+// (include/turboquant/space_turbo_quant.h:207-230).
+struct SpaceDev {
+  int dim;          // input dim
+  int D;            // padded dim: next power of two >= max(dim, 4)
+  int log2D;
+  int bits;         // bits per coordinate, 2..9
+  int mse_bits;     // bits - 1
+  int nib;          // bits <= 4: two 4-bit units per byte
+  int packed;       // packed bytes per code
+  int cs;           // code size = packed + 12
+  int L;            // 2^(bits-1) centroids
+  float scale;      // sqrt(pi/2)/sqrt(D), as the reference computes it in fp32
+  float inv_sqrt_D; // 1.0f / sqrtf((float)D), correctly rounded
+  const float *cent;  // [L]   Lloyd-Max centroids (ascending)
+  const float *bnd;   // [L-1] decision boundaries (ascending)
+  const float *rot;   // [D]   +-1 rotation signs (rot_seed)
+  const float *qjl;   // [D]   +-1 QJL signs (qjl_seed)
+};
+
+// fp32 trailer field at any byte alignment (4-bit codes at D = 4 are 14 bytes
+// long, so their trailers are only 2-byte aligned).
+__device__ inline float load_f32(const uint8_t *p, bool aligned) {
+  if (aligned) return *reinterpret_cast<const float *>(p);
+  uint32_t v = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
+               ((uint32_t)p[3] << 24);
+  return __uint_as_float(v);
+}
+
+// Prepared query layout: [q_rot (D) | s_q (D) | q_norm_sq | q_norm | 0 | 0].
+__host__ __device__ inline long prep_stride(int D) { return 2L * D + 4; }
+
+// ---- launchers (defined in the .hip files) ------------------------------
+hipError_t launch_encode(const SpaceDev &s, const float *X, long n,
+                         uint8_t *codes, hipStream_t st);
+hipError_t launch_prepare(const SpaceDev &s, const float *Q, long m,
+                          float *prep, hipStream_t st);
+hipError_t launch_prepare_from_codes(const SpaceDev &s, const uint8_t *codes,
+                                     long m, float *prep, hipStream_t st);
+// Plain fp32 scan: out[q*ldo + j] for q < m, j < n.
+hipError_t launch_scan_fp32(const SpaceDev &s, const float *prep, long m,
+                            const uint8_t *codes, long n, float *out,
+                            bool symmetric, hipStream_t st);
+bool encode_supported(int D);
+bool scan_supported(int D);
+
+}  // namespace tq

@@ -1,0 +1,378 @@
+// tq_encode.hip -- TurboQuant encoder and query preparation for gfx950.
+//
+// Follows encodeVector (include/turboquant/space_turbo_quant.h:339-396) and
+// prepareQuery (:439-460). Codes must be byte-identical to the pinned
+// reference build, so every floating-point step reproduces that build:
+//   * this file is compiled with -ffp-contract=off; the one multiply-add the
+//     reference build fuses (`rotated - centroid*sigma`, :366) is an explicit
+//     fmaf, everything else is a single correctly rounded IEEE op;
+//   * the three `s += x*x` reductions (:342-344, :353-355, :369-371) follow
+//     the order that build vectorises them in: eight FMA chains over the
+//     residues mod 8, folded ((a0+a4)+(a2+a6))+((a1+a5)+(a3+a7)), then a
+//     4-wide block of plain products folded (m0+m2)+(m1+m3), then scalar FMA;
+//   * the Walsh-Hadamard butterflies run in ascending step order
+//     (include/turboquant/turbo_quant.h:46-60) followed by one multiply by
+//     fl(1/sqrt(D)) (:68-71);
+//   * division and sqrt are HIP's correctly rounded fp32 forms.
+//
+// Layout: one vector is owned by G lanes (G = 8, 16, 32 or 64) of a
+// wavefront; lane (h, j) (j = lane & 7, h = (lane >> 3) & (G/8 - 1)) holds
+// coordinates i = 8*(h*TP + t) + j for t < TP in registers, D = G*TP. Each
+// lane therefore owns whole FMA chains of one residue class mod 8, which is
+// what makes the reference's reduction order cheap to reproduce: the chain
+// runs in registers, the 8-way fold is three cross-lane adds. Butterfly
+// steps 1, 2, 4 cross lanes j, steps 8..8*TP/2 stay in registers, larger
+// steps cross lanes h.
+#include "tq_common.h"
+
+namespace tq {
+namespace {
+
+template <int TP>
+__device__ __forceinline__ void lane_butterfly(float (&r)[TP], int m, int lane) {
+  const bool hi = (lane & m) != 0;
+#pragma unroll
+  for (int t = 0; t < TP; ++t) {
+    float p = __shfl_xor(r[t], m);
+    r[t] = hi ? (p - r[t]) : (r[t] + p);
+  }
+}
+
+// Unnormalised WHT over the D = G*TP coordinates held by a lane group, in
+// ascending step order.
+template <int G, int TP>
+__device__ __forceinline__ void wht_group(float (&r)[TP], int lane) {
+#pragma unroll
+  for (int m = 1; m < 8; m <<= 1) lane_butterfly(r, m, lane);
+#pragma unroll
+  for (int st = 1; st < TP; st <<= 1) {
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+      if ((t & st) == 0) {
+        float a = r[t], b = r[t + st];
+        r[t] = a + b;
+        r[t + st] = a - b;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 8; m < G; m <<= 1) lane_butterfly(r, m, lane);
+}
+
+// Sum of squares over the first `nblocks` blocks of 8 coordinates, in the
+// reference build's order (see file header). Chains of residue j run through
+// lanes h = 0, 1, ... in turn; every lane of the group returns the sum.
+template <int G, int TP>
+__device__ __forceinline__ float chain_sumsq(const float (&r)[TP], int nblocks,
+                                             int h, int j, int lane, int grp0) {
+  constexpr int HG = G / 8;
+  float acc = 0.0f;
+#pragma unroll
+  for (int hh = 0; hh < HG; ++hh) {
+    if (h == hh) {
+#pragma unroll
+      for (int t = 0; t < TP; ++t)
+        if (hh * TP + t < nblocks) acc = fmaf(r[t], r[t], acc);
+    }
+    if (hh + 1 < HG) {
+      float a = __shfl(acc, (lane - 8) & 63);
+      if (h == hh + 1) acc = a;
+    }
+  }
+  float a = __shfl(acc, grp0 + (HG - 1) * 8 + j);
+  float hs = a + __shfl_xor(a, 4);
+  float g = hs + __shfl_xor(hs, 2);
+  return g + __shfl_xor(g, 1);
+}
+
+// The part of the norm loop the reference build does not vectorise 8-wide:
+// a 4-wide block of plain products, then scalar FMA (elements i0 .. n-1).
+__device__ __forceinline__ float tail_sumsq(float s, const float *x, int i0, int n) {
+  int i = i0;
+  if (n - i >= 4) {
+    float m0 = x[i] * x[i], m1 = x[i + 1] * x[i + 1];
+    float m2 = x[i + 2] * x[i + 2], m3 = x[i + 3] * x[i + 3];
+    s = s + ((m0 + m2) + (m1 + m3));
+    i += 4;
+  }
+  for (; i < n; ++i) s = fmaf(x[i], x[i], s);
+  return s;
+}
+
+// idx = #{k : v > bnd[k]} over 2^mbits - 1 ascending boundaries (quantize,
+// space_turbo_quant.h:332-337); NaN compares false everywhere and maps to 0.
+__device__ __forceinline__ int quantize_idx(float v, const float *bnd, int mbits) {
+  int idx = 0;
+  for (int k = mbits - 1; k >= 0; --k) {
+    int step = 1 << k;
+    if (v > bnd[idx + step - 1]) idx += step;
+  }
+  return idx;
+}
+
+template <int G, int TP, bool NIB, bool PREP>
+__global__ __launch_bounds__(256) void k_encode(SpaceDev s, const float *__restrict__ X,
+                                                long n, uint8_t *__restrict__ codes,
+                                                float *__restrict__ prep) {
+  constexpr int VPW = 64 / G;
+  extern __shared__ float lds[];
+  float *l_rot = lds;
+  float *l_qjl = lds + s.D;
+  float *l_cent = l_qjl + s.D;
+  float *l_bnd = l_cent + s.L;
+  for (int i = threadIdx.x; i < s.D; i += blockDim.x) {
+    l_rot[i] = s.rot[i];
+    l_qjl[i] = s.qjl[i];
+  }
+  for (int i = threadIdx.x; i < s.L; i += blockDim.x) l_cent[i] = s.cent[i];
+  for (int i = threadIdx.x; i < s.L - 1; i += blockDim.x) l_bnd[i] = s.bnd[i];
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int j = lane & 7;
+  const int h = (lane >> 3) & (G / 8 - 1);
+  const int grp0 = lane & ~(G - 1);
+  const long vec = ((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * VPW + lane / G;
+  const bool active = vec < n;
+  const float *x = X + (active ? vec : 0L) * (long)s.dim;
+
+  float r[TP];
+#pragma unroll
+  for (int t = 0; t < TP; ++t) {
+    int i = 8 * (h * TP + t) + j;
+    r[t] = (active && i < s.dim) ? x[i] : 0.0f;
+  }
+
+  // ||x||^2 over the input dim. encodeVector (space_turbo_quant.h:342-345)
+  // is vectorised by the reference build; prepareQuery (:443-446) adds into
+  // the returned struct, which the compiler must assume aliases the query, so
+  // it stays one scalar FMA chain (run by the group's first lane).
+  float nsq;
+  if constexpr (PREP) {
+    float acc = 0.0f;
+    if (active && lane == grp0)
+      for (int i = 0; i < s.dim; ++i) acc = fmaf(x[i], x[i], acc);
+    nsq = __shfl(acc, grp0);
+  } else {
+    const int q8 = s.dim >> 3;
+    nsq = chain_sumsq<G, TP>(r, q8, h, j, lane, grp0);
+    if (active) nsq = tail_sumsq(nsq, x, q8 * 8, s.dim);
+  }
+  const float norm = sqrtf(nsq);
+  const float inv_norm = (norm > 1e-10f) ? (1.0f / norm) : 0.0f;
+
+  // rotated = RHT_rot(x / ||x||), zero-padded (:346-350, turbo_quant.h:89-99).
+#pragma unroll
+  for (int t = 0; t < TP; ++t) {
+    int i = 8 * (h * TP + t) + j;
+    r[t] = r[t] * inv_norm;
+    r[t] = r[t] * l_rot[i];
+  }
+  wht_group<G, TP>(r, lane);
+#pragma unroll
+  for (int t = 0; t < TP; ++t) r[t] = r[t] * s.inv_sqrt_D;
+
+  if constexpr (PREP) {
+    // TurboQuantPreparedQuery: q_rot, then s_q = RHT_qjl(q_rot) (:449-458).
+    float *pq = prep + (active ? vec : 0L) * prep_stride(s.D);
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < TP; ++t) pq[8 * (h * TP + t) + j] = r[t];
+    }
+#pragma unroll
+    for (int t = 0; t < TP; ++t) r[t] = r[t] * l_qjl[8 * (h * TP + t) + j];
+    wht_group<G, TP>(r, lane);
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < TP; ++t) pq[s.D + 8 * (h * TP + t) + j] = r[t] * s.inv_sqrt_D;
+      if (lane == grp0) {
+        pq[2 * s.D] = nsq;
+        pq[2 * s.D + 1] = norm;
+        pq[2 * s.D + 2] = 0.0f;
+        pq[2 * s.D + 3] = 0.0f;
+      }
+    }
+    return;
+  } else {
+    // sigma = RMS of the rotated coordinates, clamped (:352-359).
+    const float var = chain_sumsq<G, TP>(r, G / 8 * TP, h, j, lane, grp0);
+    float sigma = sqrtf(var / (float)s.D);
+    if (sigma < 1e-10f) sigma = 1e-10f;
+    const float inv_sigma = 1.0f / sigma;
+
+    // Lloyd-Max index and residual (:361-367).
+    uint32_t idxw[(TP + 3) / 4];
+#pragma unroll
+    for (int k = 0; k < (TP + 3) / 4; ++k) idxw[k] = 0u;
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+      int q = quantize_idx(r[t] * inv_sigma, l_bnd, s.mse_bits);
+      idxw[t >> 2] |= (uint32_t)q << (8 * (t & 3));
+      r[t] = fmaf(-l_cent[q], sigma, r[t]);
+    }
+    // gamma = ||residual|| (:369-372).
+    const float gamma = sqrtf(chain_sumsq<G, TP>(r, G / 8 * TP, h, j, lane, grp0));
+
+    // QJL signs of RHT_qjl(residual) (:374).
+#pragma unroll
+    for (int t = 0; t < TP; ++t) r[t] = r[t] * l_qjl[8 * (h * TP + t) + j];
+    wht_group<G, TP>(r, lane);
+
+    // Pack units (idx<<1)|sign (:375-391) and write the trailer (:393-395).
+    uint8_t *out = codes + (active ? vec : 0L) * (long)s.cs;
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+      uint32_t q = (idxw[t >> 2] >> (8 * (t & 3))) & 0xFFu;
+      uint32_t bit = (r[t] * s.inv_sqrt_D >= 0.0f) ? 1u : 0u;
+      uint32_t u = ((q << 1) | bit) & 0xFFu;
+      if constexpr (NIB) {
+        // word (h*TP + t) holds coordinates 8*(h*TP+t) .. +7, nibble j.
+        uint32_t w = (u & 0xFu) << (4 * j);
+        w |= __shfl_xor(w, 1);
+        w |= __shfl_xor(w, 2);
+        w |= __shfl_xor(w, 4);
+        if (active && (t & 7) == j)
+          *reinterpret_cast<uint32_t *>(out + 4 * (h * TP + t)) = w;
+      } else {
+        // word 2*(h*TP+t) + (j>>2) holds coordinates 4*word .. +3.
+        uint32_t w = u << (8 * (j & 3));
+        w |= __shfl_xor(w, 1);
+        w |= __shfl_xor(w, 2);
+        if (active && (t & 3) == (j & 3))
+          *reinterpret_cast<uint32_t *>(out + 4 * (2 * (h * TP + t) + (j >> 2))) = w;
+      }
+    }
+    if (active && lane == grp0) {
+      float *meta = reinterpret_cast<float *>(out + s.packed);
+      meta[0] = norm;
+      meta[1] = gamma;
+      meta[2] = sigma;
+    }
+  }
+}
+
+// D == 4 (input dims 1..4): one thread per vector, the same arithmetic
+// written serially (the 8-wide chains are empty at this size).
+template <bool PREP>
+__global__ __launch_bounds__(256) void k_encode_d4(SpaceDev s, const float *__restrict__ X,
+                                                   long n, uint8_t *__restrict__ codes,
+                                                   float *__restrict__ prep) {
+  const long vec = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (vec >= n) return;
+  const float *x = X + vec * s.dim;
+  float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int i = 0; i < s.dim; ++i) r[i] = x[i];
+  float nsq = 0.0f;
+  if constexpr (PREP) {
+    for (int i = 0; i < s.dim; ++i) nsq = fmaf(x[i], x[i], nsq);
+  } else {
+    nsq = tail_sumsq(0.0f, x, 0, s.dim);
+  }
+  const float norm = sqrtf(nsq);
+  const float inv_norm = (norm > 1e-10f) ? (1.0f / norm) : 0.0f;
+  for (int i = 0; i < 4; ++i) r[i] = (r[i] * inv_norm) * s.rot[i];
+  auto wht4 = [&](float *v) {
+    for (int st = 1; st < 4; st <<= 1)
+      for (int i = 0; i < 4; i += 2 * st)
+        for (int k = 0; k < st; ++k) {
+          float a = v[i + k], b = v[i + st + k];
+          v[i + k] = a + b;
+          v[i + st + k] = a - b;
+        }
+    for (int i = 0; i < 4; ++i) v[i] = v[i] * s.inv_sqrt_D;
+  };
+  wht4(r);
+  if constexpr (PREP) {
+    float *pq = prep + vec * prep_stride(4);
+    for (int i = 0; i < 4; ++i) pq[i] = r[i];
+    for (int i = 0; i < 4; ++i) r[i] = r[i] * s.qjl[i];
+    wht4(r);
+    for (int i = 0; i < 4; ++i) pq[4 + i] = r[i];
+    pq[8] = nsq;
+    pq[9] = norm;
+    pq[10] = 0.0f;
+    pq[11] = 0.0f;
+  } else {
+    float var = tail_sumsq(0.0f, r, 0, 4);
+    float sigma = sqrtf(var / 4.0f);
+    if (sigma < 1e-10f) sigma = 1e-10f;
+    const float inv_sigma = 1.0f / sigma;
+    int idx[4];
+    for (int i = 0; i < 4; ++i) {
+      idx[i] = quantize_idx(r[i] * inv_sigma, s.bnd, s.mse_bits);
+      r[i] = fmaf(-s.cent[idx[i]], sigma, r[i]);
+    }
+    const float gamma = sqrtf(tail_sumsq(0.0f, r, 0, 4));
+    for (int i = 0; i < 4; ++i) r[i] = r[i] * s.qjl[i];
+    wht4(r);
+    uint8_t *out = codes + vec * s.cs;
+    uint32_t u[4];
+    for (int i = 0; i < 4; ++i)
+      u[i] = ((uint32_t)(idx[i] << 1) | (r[i] >= 0.0f ? 1u : 0u)) & 0xFFu;
+    if (s.nib) {
+      out[0] = (uint8_t)(u[0] | (u[1] << 4));
+      out[1] = (uint8_t)(u[2] | (u[3] << 4));
+    } else {
+      for (int i = 0; i < 4; ++i) out[i] = (uint8_t)u[i];
+    }
+    float meta[3] = {norm, gamma, sigma};
+    for (int k = 0; k < 12; ++k) out[s.packed + k] = reinterpret_cast<uint8_t *>(meta)[k];
+  }
+}
+
+template <int G, int TP, bool PREP>
+hipError_t launch_g(const SpaceDev &s, const float *X, long n, uint8_t *codes, float *prep,
+                    hipStream_t st) {
+  constexpr int threads = 256;
+  const long per_block = (threads / 64) * (64 / G);
+  const long blocks = (n + per_block - 1) / per_block;
+  const size_t lds = sizeof(float) * (2 * (size_t)s.D + 2 * (size_t)s.L);
+  if (s.nib)
+    hipLaunchKernelGGL((k_encode<G, TP, true, PREP>), dim3((unsigned)blocks), dim3(threads), lds,
+                       st, s, X, n, codes, prep);
+  else
+    hipLaunchKernelGGL((k_encode<G, TP, false, PREP>), dim3((unsigned)blocks), dim3(threads), lds,
+                       st, s, X, n, codes, prep);
+  return hipGetLastError();
+}
+
+template <bool PREP>
+hipError_t dispatch(const SpaceDev &s, const float *X, long n, uint8_t *codes, float *prep,
+                    hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  switch (s.D) {
+    case 4: {
+      const long blocks = (n + 255) / 256;
+      hipLaunchKernelGGL((k_encode_d4<PREP>), dim3((unsigned)blocks), dim3(256), 0, st, s, X, n,
+                         codes, prep);
+      return hipGetLastError();
+    }
+    case 8: return launch_g<8, 1, PREP>(s, X, n, codes, prep, st);
+    case 16: return launch_g<8, 2, PREP>(s, X, n, codes, prep, st);
+    case 32: return launch_g<8, 4, PREP>(s, X, n, codes, prep, st);
+    case 64: return launch_g<8, 8, PREP>(s, X, n, codes, prep, st);
+    case 128: return launch_g<8, 16, PREP>(s, X, n, codes, prep, st);
+    case 256: return launch_g<8, 32, PREP>(s, X, n, codes, prep, st);
+    case 512: return launch_g<8, 64, PREP>(s, X, n, codes, prep, st);
+    case 1024: return launch_g<16, 64, PREP>(s, X, n, codes, prep, st);
+    case 2048: return launch_g<32, 64, PREP>(s, X, n, codes, prep, st);
+    case 4096: return launch_g<64, 64, PREP>(s, X, n, codes, prep, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+bool encode_supported(int D) { return D >= 4 && D <= 4096; }
+
+hipError_t launch_encode(const SpaceDev &s, const float *X, long n, uint8_t *codes,
+                         hipStream_t st) {
+  return dispatch<false>(s, X, n, codes, nullptr, st);
+}
+
+hipError_t launch_prepare(const SpaceDev &s, const float *Q, long m, float *prep,
+                          hipStream_t st) {
+  return dispatch<true>(s, Q, m, nullptr, prep, st);
+}
+
+}  // namespace tq

@@ -1,0 +1,12 @@
+// tq_mfma.h -- fp16 hi+lo MFMA form of the m x n asymmetric scan.
+#pragma once
+#include "tq_common.h"
+
+namespace tq {
+// out[q*n + j] for q < m, j < n from prepared queries (prep_stride floats
+// each) and codes; returns hipErrorNotSupported for padded dims it does not
+// handle.
+hipError_t launch_scan_mfma(const SpaceDev &s, const float *prep, long m,
+                            const uint8_t *codes, long n, float *out,
+                            hipStream_t st);
+}  // namespace tq

@@ -1,0 +1,152 @@
+"""ctypes binding of the C ABI in include/tq_hip.h (libtq_hip.so).
+
+This is the binding a Python host would add to call the library without the
+pybind11 module (tests and benchmarks use it to drive device-resident
+buffers). It raises ImportError when the library has not been built: there is
+no fallback path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtq_hip.so")
+
+TQ_OK = 0
+TQ_EINVAL = -1
+TQ_EHIP = -2
+TQ_ENOMEM = -3
+TQ_EUNSUPPORTED = -4
+TQ_MEM_HOST = 0
+TQ_MEM_DEVICE = 1
+TQ_MODE_FP32 = 0
+TQ_MODE_MFMA = 1
+
+EXPORTED = [
+    "tq_last_error", "tq_device_count", "tq_space_create", "tq_space_destroy",
+    "tq_space_info_get", "tq_space_tables", "tq_encode_batch", "tq_prepare_queries",
+    "tq_distance_1_to_n", "tq_distance_m_to_n", "tq_distance_prepared",
+    "tq_distance_m_to_n_symmetric",
+]
+
+
+class SpaceInfo(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_uint64),
+        ("padded_dim", ctypes.c_uint64),
+        ("code_size", ctypes.c_uint64),
+        ("packed_bytes", ctypes.c_uint64),
+        ("bits_per_coord", ctypes.c_int32),
+        ("num_levels", ctypes.c_int32),
+        ("scale", ctypes.c_float),
+        ("padded", ctypes.c_int32),
+        ("prepared_stride", ctypes.c_uint64),
+    ]
+
+
+class TQError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built (run turboquant-space_amd/build.py)")
+        L = ctypes.CDLL(LIB_PATH)
+        P, U64, I = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+        L.tq_last_error.restype = ctypes.c_char_p
+        L.tq_last_error.argtypes = []
+        L.tq_device_count.restype = I
+        L.tq_device_count.argtypes = []
+        L.tq_space_create.restype = I
+        L.tq_space_create.argtypes = [U64, I, U64, U64, ctypes.POINTER(P)]
+        L.tq_space_destroy.restype = I
+        L.tq_space_destroy.argtypes = [P]
+        L.tq_space_info_get.restype = I
+        L.tq_space_info_get.argtypes = [P, ctypes.POINTER(SpaceInfo)]
+        L.tq_space_tables.restype = I
+        L.tq_space_tables.argtypes = [P, P, P, P, P]
+        L.tq_encode_batch.restype = I
+        L.tq_encode_batch.argtypes = [P, P, U64, P, I, P]
+        L.tq_prepare_queries.restype = I
+        L.tq_prepare_queries.argtypes = [P, P, U64, P, I, P]
+        L.tq_distance_1_to_n.restype = I
+        L.tq_distance_1_to_n.argtypes = [P, P, P, U64, P, I, P]
+        L.tq_distance_m_to_n.restype = I
+        L.tq_distance_m_to_n.argtypes = [P, P, U64, P, U64, P, I, I, P]
+        L.tq_distance_prepared.restype = I
+        L.tq_distance_prepared.argtypes = [P, P, U64, P, U64, P, I, P]
+        L.tq_distance_m_to_n_symmetric.restype = I
+        L.tq_distance_m_to_n_symmetric.argtypes = [P, P, U64, P, U64, P, I, P]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != TQ_OK:
+        raise TQError(rc, lib().tq_last_error().decode())
+
+
+def ptr(x):
+    """Raw address of a numpy array, a torch tensor or an int."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    return x.ctypes.data
+
+
+class Space:
+    """Owns a tq_space handle."""
+
+    def __init__(self, dim, bits=4, rot_seed=42, qjl_seed=137):
+        h = ctypes.c_void_p()
+        check(lib().tq_space_create(dim, bits, rot_seed, qjl_seed, ctypes.byref(h)))
+        self.h = h
+        info = SpaceInfo()
+        check(lib().tq_space_info_get(self.h, ctypes.byref(info)))
+        self.info = info
+        self.dim = int(info.dim)
+        self.D = int(info.padded_dim)
+        self.cs = int(info.code_size)
+        self.bits = bits
+        self.prep_stride = int(info.prepared_stride)
+
+    def close(self):
+        if self.h:
+            lib().tq_space_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def encode(self, X, n, codes, mem=TQ_MEM_DEVICE, stream=None):
+        check(lib().tq_encode_batch(self.h, ptr(X), n, ptr(codes), mem, stream))
+
+    def prepare(self, Q, m, prep, mem=TQ_MEM_DEVICE, stream=None):
+        check(lib().tq_prepare_queries(self.h, ptr(Q), m, ptr(prep), mem, stream))
+
+    def scan(self, prep, m, codes, n, out, mode=TQ_MODE_FP32, stream=None):
+        check(lib().tq_distance_prepared(self.h, ptr(prep), m, ptr(codes), n, ptr(out), mode,
+                                         stream))
+
+    def m_to_n(self, Q, m, codes, n, out, mode=TQ_MODE_FP32, mem=TQ_MEM_DEVICE, stream=None):
+        check(lib().tq_distance_m_to_n(self.h, ptr(Q), m, ptr(codes), n, ptr(out), mode, mem,
+                                       stream))
+
+    def one_to_n(self, q, codes, n, out, mem=TQ_MEM_DEVICE, stream=None):
+        check(lib().tq_distance_1_to_n(self.h, ptr(q), ptr(codes), n, ptr(out), mem, stream))
+
+    def symmetric(self, A, m, B, n, out, mem=TQ_MEM_DEVICE, stream=None):
+        check(lib().tq_distance_m_to_n_symmetric(self.h, ptr(A), m, ptr(B), n, ptr(out), mem,
+                                                 stream))
