@@ -158,3 +158,13 @@ def scale_rel_err(got, want, qns, xns):
     denom = np.asarray(qns)[:, None] + np.asarray(xns)[None, :]
     denom = np.maximum(denom, 1e-30)
     return np.abs(got.astype(np.float64) - want.astype(np.float64)) / denom
+
+
+def fp32_parity_excess(got, oracle32, exact, qns, xns):
+    """How far the GPU's fp32 result is from the fp32 oracle beyond the
+    oracle's own rounding error, both measured against the fp64 restatement of
+    the estimator, in units of ||q||^2 + ||x||^2. The fp32 bar is
+    fp32_parity_excess <= 1e-5: the reference sums in fp32 too, and over long
+    same-sign sums (self-distances at large D) its own error approaches 1e-5,
+    while the GPU's tree-ordered sums stay well below it."""
+    return (scale_rel_err(got, oracle32, qns, xns) - scale_rel_err(oracle32, exact, qns, xns))

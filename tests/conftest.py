@@ -4,6 +4,11 @@ import sys
 
 import pytest
 
+try:  # share one HIP runtime with torch (see turboquant/_runtime.py)
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 REPO = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
 PKG_ROOT = os.path.join(REPO, "turboquant-space_amd")
 for p in (PKG_ROOT, os.path.join(REPO, "tests")):

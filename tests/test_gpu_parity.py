@@ -2,8 +2,9 @@
 called through the C ABI (include/tq_hip.h) on device-resident buffers.
 
 Bars: codes and prepared queries bit-identical; plain fp32 distances within
-1e-5 of (||q||^2 + ||x||^2) of the oracle; the MFMA form within 1e-4 of the
-same scale of the FP64 restatement.
+1e-5 of (||q||^2 + ||x||^2) of the oracle (beyond the oracle's own fp32
+rounding where that is measured, see fp32_parity_excess); the MFMA form within
+1e-4 of the same scale of the FP64 restatement.
 """
 import hashlib
 import json
@@ -12,7 +13,7 @@ import sys
 import numpy as np
 import pytest
 
-from _oracle import MANIFEST, Oracle, load_golden, scale_rel_err
+from _oracle import MANIFEST, Oracle, fp32_parity_excess, load_golden, scale_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -124,7 +125,9 @@ def test_random_encode_and_scan(gpu, torch_cuda, dim, bits):
     say(f"  fp32 max scale-rel err {err.max():.3e}")
     assert err.max() <= FP32_TOL
     s = gpu_symmetric(torch, sp, got[:5], got)
-    assert scale_rel_err(s, o.symmetric(got[:5], got), xns[:5], xns).max() <= FP32_TOL
+    excess = fp32_parity_excess(s, o.symmetric(got[:5], got), o.symmetric_f64(got[:5], got),
+                                xns[:5], xns)
+    assert excess.max() <= FP32_TOL
     sp.close()
 
 
@@ -207,3 +210,36 @@ def test_invalid_arguments(gpu):
     sp = capi.Space(64, 4)
     with pytest.raises(capi.TQError):
         sp.m_to_n(None, 1, None, 1, None, mode=7)
+
+
+@pytest.mark.parametrize("dim,bits,m,n", [(768, 4, 70, 1001), (768, 8, 64, 512), (1000, 4, 33, 777),
+                                          (128, 4, 130, 300), (96, 4, 5, 257), (1536, 4, 65, 100),
+                                          (16, 9, 3, 50), (4096, 8, 17, 129), (384, 2, 1, 1)])
+def test_mfma_against_fp64_estimator(gpu, torch_cuda, dim, bits, m, n):
+    say(f"[gpu] mfma dim={dim} bits={bits} m={m} n={n}")
+    capi, torch = gpu, torch_cuda
+    sp = capi.Space(dim, bits)
+    o = Oracle(dim, bits)
+    rng = np.random.default_rng(dim + m + n)
+    X = rng.standard_normal((n, dim)).astype(np.float32)
+    Q = rng.standard_normal((m, dim)).astype(np.float32)
+    Q[:1] = X[:1] * 1.01          # a near-duplicate pair: the cancellation-prone case
+    codes = o.encode(X)
+    got = gpu_m_to_n(torch, capi, sp, Q, codes, capi.TQ_MODE_MFMA)
+    want = o.m_to_n_f64(Q, codes)
+    qns = np.sum(Q.astype(np.float64) ** 2, 1)
+    xns = o.trailer(codes)[:, 0].astype(np.float64) ** 2
+    err = scale_rel_err(got, want, qns, xns)
+    say(f"  mfma max scale-rel err {err.max():.3e} mean {err.mean():.3e}")
+    assert err.max() <= MFMA_TOL
+    fp32 = gpu_m_to_n(torch, capi, sp, Q, codes, capi.TQ_MODE_FP32)
+    assert scale_rel_err(fp32, want, qns, xns).max() <= FP32_TOL
+
+
+def test_mfma_rejects_tiny_dims(gpu, torch_cuda):
+    capi, torch = gpu, torch_cuda
+    sp = capi.Space(8, 4)
+    Q = np.zeros((1, 8), np.float32)
+    codes = np.zeros((1, sp.cs), np.uint8)
+    with pytest.raises(capi.TQError):
+        gpu_m_to_n(torch, capi, sp, Q, codes, capi.TQ_MODE_MFMA)

@@ -4,6 +4,7 @@ Drop-in for the reference package: ``from turboquant import TurboQuantSpace``
 gives the same class, constructor and methods, backed by hand-written HIP
 kernels through the C ABI in include/tq_hip.h.
 """
+from . import _runtime  # noqa: F401  (must run before the native module loads)
 from ._turboquant import TurboQuantSpace, device_count
 
 __all__ = ["TurboQuantSpace", "device_count"]

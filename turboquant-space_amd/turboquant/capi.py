@@ -8,6 +8,8 @@ no fallback path.
 import ctypes
 import os
 
+from . import _runtime  # noqa: F401  (must run before libtq_hip.so loads)
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtq_hip.so")
 
