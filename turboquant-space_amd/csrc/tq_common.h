@@ -51,6 +51,11 @@ hipError_t launch_prepare_from_codes(const SpaceDev &s, const uint8_t *codes,
 hipError_t launch_scan_fp32(const SpaceDev &s, const float *prep, long m,
                             const uint8_t *codes, long n, float *out,
                             bool symmetric, hipStream_t st);
+// 4-bit codes, 32 <= D <= 1024: pair-table scan (tq_lut.hip).
+bool lut_supported(const SpaceDev &s);
+hipError_t launch_scan_lut(const SpaceDev &s, const float *prep, long m,
+                           const uint8_t *codes, long n, float *out,
+                           bool symmetric, hipStream_t st);
 bool encode_supported(int D);
 bool scan_supported(int D);
 

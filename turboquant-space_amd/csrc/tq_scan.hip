@@ -175,6 +175,7 @@ bool scan_supported(int D) { return D >= 4 && D <= 4096; }
 hipError_t launch_scan_fp32(const SpaceDev &s, const float *prep, long m, const uint8_t *codes,
                             long n, float *out, bool symmetric, hipStream_t st) {
   if (m <= 0 || n <= 0) return hipSuccess;
+  if (lut_supported(s)) return launch_scan_lut(s, prep, m, codes, n, out, symmetric, st);
   switch (s.D) {
     case 4: return launch_cpl<4>(s, prep, m, codes, n, out, symmetric, st);
     case 8: return launch_cpl<8>(s, prep, m, codes, n, out, symmetric, st);
