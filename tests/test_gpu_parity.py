@@ -214,7 +214,7 @@ def test_invalid_arguments(gpu):
 
 @pytest.mark.parametrize("dim,bits,m,n", [(768, 4, 70, 1001), (768, 8, 64, 512), (1000, 4, 33, 777),
                                           (128, 4, 130, 300), (96, 4, 5, 257), (1536, 4, 65, 100),
-                                          (16, 9, 3, 50), (4096, 8, 17, 129), (384, 2, 1, 1)])
+                                          (32, 9, 3, 50), (4096, 8, 17, 129), (384, 2, 1, 1)])
 def test_mfma_against_fp64_estimator(gpu, torch_cuda, dim, bits, m, n):
     say(f"[gpu] mfma dim={dim} bits={bits} m={m} n={n}")
     capi, torch = gpu, torch_cuda
@@ -237,9 +237,10 @@ def test_mfma_against_fp64_estimator(gpu, torch_cuda, dim, bits, m, n):
 
 
 def test_mfma_rejects_tiny_dims(gpu, torch_cuda):
+    # the MFMA form walks the padded dim in 32-coordinate chunks
     capi, torch = gpu, torch_cuda
-    sp = capi.Space(8, 4)
-    Q = np.zeros((1, 8), np.float32)
+    sp = capi.Space(16, 4)
+    Q = np.zeros((1, 16), np.float32)
     codes = np.zeros((1, sp.cs), np.uint8)
     with pytest.raises(capi.TQError):
         gpu_m_to_n(torch, capi, sp, Q, codes, capi.TQ_MODE_MFMA)

@@ -11,8 +11,10 @@
 // fp32 accuracy, so both are split into hi + lo fp16 parts (v = hi + lo with
 // hi = fp16(v), lo = fp16(v - hi), ~22 significant bits) and the MSE term is
 // accumulated as q_hi*c_hi + q_lo*c_hi + q_hi*c_lo (the dropped q_lo*c_lo is
-// ~2^-22 relative); the +-1 QJL factors are exact in fp16, so that term is
-// s_hi*t + s_lo*t. Everything accumulates in fp32 (v_mfma_f32_32x32x16_f16).
+// ~2^-22 relative). The +-1 QJL factors are exact in fp16 and the QJL term
+// enters the distance with a factor scale*gamma ~ 1e-2, so s_q is used at
+// fp16 (one pass; its rounding moves distances by ~1e-7 of their scale).
+// Everything accumulates in fp32 (v_mfma_f32_32x32x16_f16).
 // Queries are scaled by 2^10 and centroids by 2^8 before the split so the lo
 // parts stay in fp16's normal range; the epilogue removes the exact powers of
 // two. Measured error against the fp64 restatement of the estimator is held
@@ -20,12 +22,11 @@
 //
 // Tiling: a workgroup (4 wavefronts) owns 64 queries x 256 codes; each
 // wavefront owns 64 queries x 64 codes as 2 x 2 blocks of 32 x 32 MFMA
-// outputs, walking the padded dim in steps of 16. Per step a lane loads one
-// 32-bit word of code bytes per 32-code block (8 coordinates of one code:
-// exactly the B-operand slice it contributes) and decodes it in registers
-// against a (c_hi | c_lo << 16) table in LDS; the A operands (query hi/lo
-// planes, written once per batch by k_split_queries) are 16-byte loads that
-// stay in L2.
+// outputs, walking the padded dim in steps of 16. The query hi/lo planes
+// (written once per batch by k_split_queries) and the code bytes are staged
+// through LDS in chunks of 64 coordinates (see k_mfma); a lane decodes the one
+// 32-bit word of code bytes it contributes per step (8 coordinates of one
+// code) in registers, against a (c_hi | c_lo << 16) table in LDS.
 #include "tq_mfma.h"
 
 namespace tq {
@@ -43,22 +44,81 @@ __device__ __forceinline__ uint32_t h_bits(_Float16 h) {
   return (uint32_t)__builtin_bit_cast(uint16_t, h);
 }
 
-// planes: [4][m][D] fp16 = q_hi, q_lo, s_hi, s_lo (scaled by 2^10).
+// planes: [3][m][D] fp16 = q_hi, q_lo, s_hi (scaled by 2^10). With
+// `nib_order`, each group of 8 coordinates is stored in the order the nibble
+// decoder emits them: 0, 2, 4, 6, 1, 3, 5, 7 (the contraction does not care
+// about the order within a group as long as both operands agree).
 __global__ __launch_bounds__(256) void k_split_queries(SpaceDev s, const float *__restrict__ prep,
-                                                       long m, _Float16 *__restrict__ planes) {
+                                                       long m, _Float16 *__restrict__ planes,
+                                                       int nib_order) {
   const long stride = prep_stride(s.D);
   const long mD = m * (long)s.D;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < mD;
        e += (long)gridDim.x * blockDim.x) {
     const long q = e / s.D;
-    const int i = (int)(e - q * s.D);
+    const int slot = (int)(e - q * s.D);
+    const int g = slot & ~7, k = slot & 7;
+    const int i = nib_order ? g + (k < 4 ? 2 * k : 2 * (k - 4) + 1) : slot;
     const float v = prep[q * stride + i] * kQScale;
     const float w = prep[q * stride + s.D + i] * kQScale;
-    const _Float16 vh = (_Float16)v, wh = (_Float16)w;
+    const _Float16 vh = (_Float16)v, wh = (_Float16)w;  // s_q needs no lo part
     planes[e] = vh;
     planes[mD + e] = (_Float16)(v - (float)vh);
     planes[2 * mD + e] = wh;
-    planes[3 * mD + e] = (_Float16)(w - (float)wh);
+  }
+}
+
+// Byte tables for v_perm decoding of 3-bit centroid indices: table t holds
+// one byte of the 8 fp16 values as two 32-bit words (indices 0-3 | 4-7).
+struct PermTab {
+  uint32_t hl_lo, hl_hi;  // low bytes of c_hi
+  uint32_t hh_lo, hh_hi;  // high bytes of c_hi
+  uint32_t ll_lo, ll_hi;  // low bytes of c_lo
+  uint32_t lh_lo, lh_hi;  // high bytes of c_lo
+};
+
+// 8 nibble units in one code word -> B fragments in the order
+// (0, 2, 4, 6, 1, 3, 5, 7): bytes of the even (odd) coordinates' values are
+// looked up four at a time, then interleaved into fp16 pairs.
+__device__ __forceinline__ void decode_nib_word(uint32_t w, const PermTab &t, h8 &bh, h8 &bl,
+                                                h8 &bt) {
+  const uint32_t ie = (w >> 1) & 0x07070707u;  // indices of coordinates 0, 2, 4, 6
+  const uint32_t io = (w >> 5) & 0x07070707u;  // indices of coordinates 1, 3, 5, 7
+  uint32_t r[4];
+  {
+    const uint32_t Le = __builtin_amdgcn_perm(t.hl_hi, t.hl_lo, ie);
+    const uint32_t He = __builtin_amdgcn_perm(t.hh_hi, t.hh_lo, ie);
+    const uint32_t Lo = __builtin_amdgcn_perm(t.hl_hi, t.hl_lo, io);
+    const uint32_t Ho = __builtin_amdgcn_perm(t.hh_hi, t.hh_lo, io);
+    r[0] = __builtin_amdgcn_perm(He, Le, 0x05010400u);
+    r[1] = __builtin_amdgcn_perm(He, Le, 0x07030602u);
+    r[2] = __builtin_amdgcn_perm(Ho, Lo, 0x05010400u);
+    r[3] = __builtin_amdgcn_perm(Ho, Lo, 0x07030602u);
+    bh = __builtin_bit_cast(h8, r);
+  }
+  {
+    const uint32_t Le = __builtin_amdgcn_perm(t.ll_hi, t.ll_lo, ie);
+    const uint32_t He = __builtin_amdgcn_perm(t.lh_hi, t.lh_lo, ie);
+    const uint32_t Lo = __builtin_amdgcn_perm(t.ll_hi, t.ll_lo, io);
+    const uint32_t Ho = __builtin_amdgcn_perm(t.lh_hi, t.lh_lo, io);
+    r[0] = __builtin_amdgcn_perm(He, Le, 0x05010400u);
+    r[1] = __builtin_amdgcn_perm(He, Le, 0x07030602u);
+    r[2] = __builtin_amdgcn_perm(Ho, Lo, 0x05010400u);
+    r[3] = __builtin_amdgcn_perm(Ho, Lo, 0x07030602u);
+    bl = __builtin_bit_cast(h8, r);
+  }
+  {
+    // fp16 -1 / +1 have high bytes 0xBC / 0x3C and low bytes 0; selector
+    // value 12 yields a zero byte.
+    const uint32_t te = w & 0x01010101u;
+    const uint32_t to = (w >> 4) & 0x01010101u;
+    const uint32_t Te = __builtin_amdgcn_perm(0u, 0x3CBCu, te);
+    const uint32_t To = __builtin_amdgcn_perm(0u, 0x3CBCu, to);
+    r[0] = __builtin_amdgcn_perm(Te, 0u, 0x050C040Cu);
+    r[1] = __builtin_amdgcn_perm(Te, 0u, 0x070C060Cu);
+    r[2] = __builtin_amdgcn_perm(To, 0u, 0x050C040Cu);
+    r[3] = __builtin_amdgcn_perm(To, 0u, 0x070C060Cu);
+    bt = __builtin_bit_cast(h8, r);
   }
 }
 
@@ -79,38 +139,141 @@ __device__ __forceinline__ void decode8(const uint32_t (&u)[8], const uint32_t *
   bt = __builtin_bit_cast(h8, pt);
 }
 
+// One workgroup: 64 queries x 256 codes. The padded dim is walked in chunks
+// of 64 coordinates; per chunk the query hi/lo planes of the 64 queries
+// (4 planes x 64 rows x 64 fp16) and the code bytes of the 256 codes are
+// staged in LDS by all 256 threads (double-buffered: the loads of chunk c+1
+// are in flight while chunk c runs on the matrix cores, one barrier per
+// chunk). Rows are padded (A: 144 B, codes: 36 / 68 B) so the fragment reads
+// of a wavefront fall on distinct LDS banks.
+constexpr int kTQ = 64;            // queries per workgroup
+constexpr int kTN = 256;           // codes per workgroup (64 per wavefront)
+constexpr int kKC = 32;            // coordinates per chunk
+constexpr int kARow = kKC + 8;     // padded A row, in halves (80 B)
+constexpr int kPlanes = 3;         // q_hi, q_lo, s_hi
+
 template <bool NIB>
-__global__ __launch_bounds__(256) void k_mfma(SpaceDev s, const _Float16 *__restrict__ planes,
+struct MfmaShape {
+  static constexpr int CB = NIB ? kKC / 2 : kKC;  // code bytes per chunk
+  static constexpr int CROW = CB + 4;              // padded code row, bytes
+  static constexpr int CPARTS = CB / 16;           // 16-byte loads per code
+};
+
+// Two workgroups per CU (two wavefronts per SIMD) so one wavefront's MFMA
+// chain overlaps the other's decode, LDS traffic and barrier waits.
+template <bool NIB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma(SpaceDev s, const _Float16 *__restrict__ planes,
                                               const float *__restrict__ prep, long m,
                                               const uint8_t *__restrict__ codes, long n,
                                               float *__restrict__ out) {
+  using Sh = MfmaShape<NIB>;
+  constexpr int CB = Sh::CB, CROW = Sh::CROW, CPARTS = Sh::CPARTS;
+  constexpr int ALOADS = kPlanes * kTQ * (kKC / 8) / 256;  // 16-byte A loads per thread
+  constexpr int CLOADS = kTN * CPARTS / 256;         // 16-byte code loads per thread
+  __shared__ _Float16 a_lds[2][kPlanes][kTQ][kARow];
+  __shared__ uint32_t c_lds[2][kTN * CROW / 4];
   __shared__ uint32_t l_tab[256];
-  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+  __shared__ float l_qn[kTQ][2];  // q_norm_sq, q_norm of the tile's queries
+
+  const int tid = threadIdx.x;
+  if (tid < kTQ) {
+    const long q = (long)blockIdx.y * kTQ + tid;
+    const float *pq = prep + (q < m ? q : 0L) * prep_stride(s.D);
+    l_qn[tid][0] = pq[2 * s.D];
+    l_qn[tid][1] = pq[2 * s.D + 1];
+  }
+  for (int i = tid; i < 256; i += blockDim.x) {
     const float c = i < s.L ? s.cent[i] * kCScale : 0.0f;
     const _Float16 ch = (_Float16)c;
     const _Float16 cl = (_Float16)(c - (float)ch);
     l_tab[i] = h_bits(ch) | (h_bits(cl) << 16);
   }
-  __syncthreads();
 
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int lr = lane & 31, kg = lane >> 5;
-  const long q0 = (long)blockIdx.y * 64;
-  const long j0 = ((long)blockIdx.x * 4 + wave) * 64;
-  const long mD = m * (long)s.D;
-
-  const _Float16 *arow[2];
-  bool qok[2];
-  const uint8_t *crow[2];
+  PermTab ptab{};
+  if constexpr (NIB) {
+    uint32_t bytes[4][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}};
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const long q = q0 + 32 * r + lr;
-    qok[r] = q < m;
-    arow[r] = planes + (qok[r] ? q : 0L) * s.D + 8 * kg;
-    const long j = j0 + 32 * r + lr;
-    crow[r] = codes + (j < n ? j : 0L) * (long)s.cs;
+    for (int k = 0; k < 8; ++k) {
+      const float c = k < s.L ? s.cent[k] * kCScale : 0.0f;
+      const _Float16 chh = (_Float16)c;
+      const _Float16 cll = (_Float16)(c - (float)chh);
+      const uint32_t hb = h_bits(chh), lb = h_bits(cll);
+      const int word = k >> 2, sh = 8 * (k & 3);
+      bytes[0][word] |= (hb & 0xFFu) << sh;
+      bytes[1][word] |= (hb >> 8) << sh;
+      bytes[2][word] |= (lb & 0xFFu) << sh;
+      bytes[3][word] |= (lb >> 8) << sh;
+    }
+    ptab = PermTab{bytes[0][0], bytes[0][1], bytes[1][0], bytes[1][1],
+                   bytes[2][0], bytes[2][1], bytes[3][0], bytes[3][1]};
   }
+
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int lr = lane & 31, kg = lane >> 5;
+  const long q0 = (long)blockIdx.y * kTQ;
+  const long j0 = (long)blockIdx.x * kTN;
+  const long mD = m * (long)s.D;
+  const int nchunks = s.D / kKC;
+
+  // Per-thread load sources and LDS destinations, fixed for the whole walk
+  // over the padded dim; a chunk only adds its coordinate offset.
+  const _Float16 *asrc[ALOADS];
+  bool aok[ALOADS];
+  int adst[ALOADS];
+#pragma unroll
+  for (int i = 0; i < ALOADS; ++i) {
+    const int idx = i * 256 + tid;
+    const int p = idx / (kTQ * kKC / 8);
+    const int row = (idx / (kKC / 8)) % kTQ;
+    const int part = idx % (kKC / 8);
+    const long q = q0 + row;
+    aok[i] = q < m;
+    asrc[i] = planes + p * mD + (aok[i] ? q : 0L) * s.D + 8 * part;
+    adst[i] = (p * kTQ + row) * kARow + 8 * part;
+  }
+  const uint8_t *csrc[CLOADS];
+  bool cok[CLOADS];
+  int cdst[CLOADS];
+#pragma unroll
+  for (int i = 0; i < CLOADS; ++i) {
+    const int idx = i * 256 + tid;
+    const int c = idx / CPARTS, part = idx % CPARTS;
+    const long j = j0 + c;
+    cok[i] = j < n;
+    csrc[i] = codes + (cok[i] ? j : 0L) * (long)s.cs + 16 * part;
+    cdst[i] = (c * CROW + 16 * part) / 4;
+  }
+
+  uint4 ra[ALOADS], rc[CLOADS];
+  auto load_chunk = [&](int ch) {
+    const int k0 = ch * kKC;
+#pragma unroll
+    for (int i = 0; i < ALOADS; ++i)
+      ra[i] = aok[i] ? *reinterpret_cast<const uint4 *>(asrc[i] + k0) : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int i = 0; i < CLOADS; ++i) {
+      if (cok[i]) {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(csrc[i] + (NIB ? k0 / 2 : k0));
+        rc[i] = make_uint4(src[0], src[1], src[2], src[3]);
+      } else {
+        rc[i] = make_uint4(0u, 0u, 0u, 0u);
+      }
+    }
+  };
+  auto store_chunk = [&](int b) {
+    _Float16 *abuf = &a_lds[b][0][0][0];
+#pragma unroll
+    for (int i = 0; i < ALOADS; ++i) *reinterpret_cast<uint4 *>(abuf + adst[i]) = ra[i];
+#pragma unroll
+    for (int i = 0; i < CLOADS; ++i) {
+      uint32_t *dst = &c_lds[b][cdst[i]];
+      dst[0] = rc[i].x;
+      dst[1] = rc[i].y;
+      dst[2] = rc[i].z;
+      dst[3] = rc[i].w;
+    }
+  };
 
   f16v acc_m[2][2], acc_q[2][2];
 #pragma unroll
@@ -123,70 +286,76 @@ __global__ __launch_bounds__(256) void k_mfma(SpaceDev s, const _Float16 *__rest
         acc_q[r][c][v] = 0.0f;
       }
 
-  const h8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int nkb = s.D / 16;
-  for (int kb = 0; kb < nkb; ++kb) {
-    h8 a[2][4];
+  load_chunk(0);
+  store_chunk(0);
+  __syncthreads();
+  for (int ch = 0; ch < nchunks; ++ch) {
+    const int b = ch & 1;
+    if (ch + 1 < nchunks) load_chunk(ch + 1);
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int kb = 0; kb < kKC / 16; ++kb) {
+      h8 a[2][kPlanes];
 #pragma unroll
-      for (int p = 0; p < 4; ++p)
-        a[r][p] = qok[r] ? *reinterpret_cast<const h8 *>(arow[r] + p * mD + kb * 16) : zero8;
-    h8 bh[2], bl[2], bt[2];
+      for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      uint32_t u[8];
-      if constexpr (NIB) {
-        const uint32_t w = *reinterpret_cast<const uint32_t *>(crow[c] + kb * 8 + 4 * kg);
-#pragma unroll
-        for (int v = 0; v < 8; ++v) u[v] = (w >> (4 * v)) & 0xFu;
-      } else {
-        const uint32_t *wp = reinterpret_cast<const uint32_t *>(crow[c] + kb * 16 + 8 * kg);
-        const uint32_t w0 = wp[0], w1 = wp[1];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          u[v] = (w0 >> (8 * v)) & 0xFFu;
-          u[4 + v] = (w1 >> (8 * v)) & 0xFFu;
-        }
-      }
-      decode8(u, l_tab, bh[c], bl[c], bt[c]);
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
+        for (int p = 0; p < kPlanes; ++p)
+          a[r][p] = *reinterpret_cast<const h8 *>(&a_lds[b][p][32 * r + lr][kb * 16 + 8 * kg]);
+      h8 bh[2], bl[2], bt[2];
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        acc_m[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[r][0], bh[c], acc_m[r][c], 0, 0, 0);
-        acc_m[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[r][1], bh[c], acc_m[r][c], 0, 0, 0);
-        acc_m[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[r][0], bl[c], acc_m[r][c], 0, 0, 0);
-        acc_q[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[r][2], bt[c], acc_q[r][c], 0, 0, 0);
-        acc_q[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[r][3], bt[c], acc_q[r][c], 0, 0, 0);
+        const int code = wave * 64 + 32 * c + lr;
+        if constexpr (NIB) {
+          decode_nib_word(c_lds[b][(code * CROW + kb * 8 + 4 * kg) / 4], ptab, bh[c], bl[c], bt[c]);
+        } else {
+          uint32_t u[8];
+          const uint32_t w0 = c_lds[b][(code * CROW + kb * 16 + 8 * kg) / 4];
+          const uint32_t w1 = c_lds[b][(code * CROW + kb * 16 + 8 * kg) / 4 + 1];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            u[v] = (w0 >> (8 * v)) & 0xFFu;
+            u[4 + v] = (w1 >> (8 * v)) & 0xFFu;
+          }
+          decode8(u, l_tab, bh[c], bl[c], bt[c]);
+        }
       }
+      // Each pass over all four 32x32 blocks before the next, so consecutive
+      // MFMAs never wait on the accumulator they just wrote.
+#define TQ_PASS(ACC, A, B)                                                               \
+  _Pragma("unroll") for (int r = 0; r < 2; ++r)                                          \
+      _Pragma("unroll") for (int c = 0; c < 2; ++c) ACC[r][c] =                          \
+          __builtin_amdgcn_mfma_f32_32x32x16_f16(a[r][A], B[c], ACC[r][c], 0, 0, 0);
+      TQ_PASS(acc_m, 0, bh)
+      TQ_PASS(acc_q, 2, bt)
+      TQ_PASS(acc_m, 1, bh)
+      TQ_PASS(acc_m, 0, bl)
+#undef TQ_PASS
+    }
+    if (ch + 1 < nchunks) store_chunk(b ^ 1);
+    __syncthreads();
   }
 
   // Epilogue: output element v of a 32x32 block sits at row
-  // 8*(v/4) + 4*kg + (v%4), column lr.
-  const long pstride = prep_stride(s.D);
-  const bool al = (s.cs & 3) == 0;
+  // 8*(v/4) + 4*kg + (v%4), column lr. The per-code factors are formed once;
+  // the expression is the reference's ip = (sigma*mse + scale*gamma*dot)*|x|*|q|,
+  // d = max(0, |q|^2 + |x|^2 - 2*ip).
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    const long j = j0 + 32 * c + lr;
+    const long j = j0 + wave * 64 + 32 * c + lr;
     if (j >= n) continue;
-    const uint8_t *cp = codes + j * (long)s.cs;
-    const float xn = load_f32(cp + s.packed, al);
-    const float gamma = load_f32(cp + s.packed + 4, al);
-    const float sigma = load_f32(cp + s.packed + 8, al);
+    const float *meta = reinterpret_cast<const float *>(codes + j * (long)s.cs + s.packed);
+    const float xn = meta[0], gamma = meta[1], sigma = meta[2];
+    const float fm = sigma * kMseUnscale;
+    const float fq = s.scale * gamma * kQjlUnscale;
+    const float xns = xn * xn;
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const long q = q0 + 32 * r + 8 * (v >> 2) + 4 * kg + (v & 3);
+        const int row = 32 * r + 8 * (v >> 2) + 4 * kg + (v & 3);
+        const long q = q0 + row;
         if (q >= m) continue;
-        const float qns = prep[q * pstride + 2 * s.D];
-        const float qn = prep[q * pstride + 2 * s.D + 1];
-        const float ip_mse = (acc_m[r][c][v] * kMseUnscale) * sigma;
-        const float corr = s.scale * gamma * (acc_q[r][c][v] * kQjlUnscale);
-        const float ip = (ip_mse + corr) * xn * qn;
-        const float d = qns + xn * xn - 2.0f * ip;
+        const float ip = (acc_m[r][c][v] * fm + acc_q[r][c][v] * fq) * xn * l_qn[row][1];
+        const float d = l_qn[row][0] + xns - 2.0f * ip;
         out[q * n + j] = d > 0.0f ? d : 0.0f;
       }
   }
@@ -196,13 +365,13 @@ __global__ __launch_bounds__(256) void k_mfma(SpaceDev s, const _Float16 *__rest
 
 hipError_t launch_scan_mfma(const SpaceDev &s, const float *prep, long m, const uint8_t *codes,
                             long n, float *out, hipStream_t st) {
-  if (s.D < 16) return hipErrorNotSupported;
+  if (s.D < kKC) return hipErrorNotSupported;
   if (m <= 0 || n <= 0) return hipSuccess;
   // grid.y covers 64 queries per workgroup row; larger batches go in chunks.
   const long chunk_max = 65535L * 64;
   const long chunk = m < chunk_max ? m : chunk_max;
   _Float16 *planes = nullptr;
-  const size_t bytes = 4 * (size_t)chunk * s.D * sizeof(_Float16);
+  const size_t bytes = kPlanes * (size_t)chunk * s.D * sizeof(_Float16);
   hipError_t e = hipMallocAsync(reinterpret_cast<void **>(&planes), bytes, st);
   if (e != hipSuccess) return e;
   const long pstride = prep_stride(s.D);
@@ -213,10 +382,10 @@ hipError_t launch_scan_mfma(const SpaceDev &s, const float *prep, long m, const 
     long blocks = (mc * (long)s.D + 255) / 256;
     if (blocks > 65535) blocks = 65535;
     hipLaunchKernelGGL(k_split_queries, dim3((unsigned)blocks), dim3(256), 0, st, s, pc, mc,
-                       planes);
+                       planes, s.nib);
     e = hipGetLastError();
     if (e != hipSuccess) break;
-    dim3 grid((unsigned)((n + 255) / 256), (unsigned)((mc + 63) / 64));
+    dim3 grid((unsigned)((n + kTN - 1) / kTN), (unsigned)((mc + kTQ - 1) / kTQ));
     if (s.nib)
       hipLaunchKernelGGL(k_mfma<true>, grid, dim3(256), 0, st, s, planes, pc, mc, codes, n, oc);
     else
