@@ -60,6 +60,18 @@ def event_time(torch, stream, fn, reps):
     return s.elapsed_time(e) / reps
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary (profiles/traffic_bench.json,
+    counters scaled by a 1 GiB copy); None if it has not been measured."""
+    path = os.path.join(REPO, "profiles", "traffic_bench.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(codes_host, Q_host, budget_s=8.0):
     """The reference CPU build (oracle/_ref, compiled in place from the
     reference sources with its Release flags) on this host's cores:
@@ -107,6 +119,9 @@ def main():
     ap.add_argument("--codes", type=int, default=N_CODES)
     ap.add_argument("--queries", type=int, default=M_QUERIES)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (a kernel of known "
+                         "traffic, used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -117,6 +132,12 @@ def main():
     from turboquant import capi
 
     torch.cuda.set_device(0)
+    if args.calibrate:
+        a = torch.empty(1 << 28, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     stream = torch.cuda.current_stream()
     sh = ctypes.c_void_p(stream.cuda_stream)
     sp = capi.Space(DIM, BITS)
@@ -183,13 +204,14 @@ def main():
         flops = 4.0 * sp.D * m * n
         achieved = flops / (r["scan_ms"] / 1e3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP16_MFMA_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP16_MFMA_PEAK_TFLOPS, "traffic": None}
+                "unit": "TFLOP/s", "frac": achieved / FP16_MFMA_PEAK_TFLOPS,
+                "traffic": measured_traffic("k_mfma")}
         dtype = "f16 hi+lo MFMA, f32 accumulate"
     else:
         bytes_mn = m * n * (cs + 4)
         achieved = bytes_mn / (r["scan_ms"] / 1e3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_lut_nib")}
         dtype = "f32"
 
     cpu = None

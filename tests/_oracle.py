@@ -46,6 +46,7 @@ def olib():
         L.tqo_wht.argtypes = [P, SZ]
         L.tqo_rht.argtypes = [P, P, SZ]
         L.tqo_encode_batch.argtypes = [P, P, SZ, P]
+        L.tqo_rotated_coords_batch.argtypes = [P, P, SZ, P]
         L.tqo_prepare_query.argtypes = [P, P, P, P, P, P]
         L.tqo_distance_1_to_n.argtypes = [P, P, P, SZ, P]
         L.tqo_distance_m_to_n.argtypes = [P, P, SZ, P, SZ, P]

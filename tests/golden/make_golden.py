@@ -56,6 +56,7 @@ def load_ref():
     L.tqref_distance_m_to_n_symmetric_full.argtypes = [P, P, SZ, P, SZ, P]
     L.tqref_distance_m_to_n_symmetric_light.argtypes = [P, P, SZ, P, SZ, P]
     L.tqref_codebook.argtypes = [ctypes.c_int, P, P]
+    L.tqref_rotated_coords_batch.argtypes = [P, P, SZ, P]
     L.tqref_signs.argtypes = [SZ, ctypes.c_uint64, P]
     return L
 
@@ -125,7 +126,10 @@ def main():
         syml = np.zeros((4, len(X)), np.float32)
         L.tqref_distance_m_to_n_symmetric_light(h, ptr(codes[-4:]), 4, ptr(codes), len(X),
                                                 ptr(syml))
+        rot = np.zeros((len(X), D), np.float32)
+        L.tqref_rotated_coords_batch(h, ptr(X), len(X), ptr(rot))
         tag = f"{dim}_{bits}"
+        out[f"rot_{tag}"] = rot
         out[f"X_{tag}"] = X
         out[f"codes_{tag}"] = codes
         out[f"Q_{tag}"] = Q

@@ -118,6 +118,17 @@ def test_prepare_matches_reference_fixture(dim, bits):
     np.testing.assert_array_equal(got.view(np.uint32), GOLD[f"prep_{dim}_{bits}"].view(np.uint32))
 
 
+@pytest.mark.parametrize("dim,bits", SHAPES)
+def test_rotated_coords_match_reference_fixture(dim, bits):
+    # rotatedCoordsBatch (space_turbo_quant.h:411-435); the reference build
+    # vectorises both of its reductions the same way as encodeVector's.
+    o = Oracle(dim, bits)
+    X = GOLD[f"X_{dim}_{bits}"]
+    got = np.zeros((len(X), o.D), np.float32)
+    olib().tqo_rotated_coords_batch(o.h, ptr(X), len(X), ptr(got))
+    np.testing.assert_array_equal(got.view(np.uint32), GOLD[f"rot_{dim}_{bits}"].view(np.uint32))
+
+
 def test_encode_full_cpu_workload_digest():
     # 100,000 x 128, 4-bit, iid N(0,1) seed 0: sha256 of the whole code matrix.
     o = Oracle(128, 4)
