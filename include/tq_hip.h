@@ -118,6 +118,69 @@ int tq_distance_m_to_n_symmetric(tq_space s, const uint8_t *codes_a,
                                  uint64_t n, float *out, int mem,
                                  void *stream);
 
+/* ---- top-k search ---------------------------------------------------------
+ * No reference counterpart: the reference returns full distance matrices and
+ * its users rank rows with numpy (argpartition + argsort,
+ * python/benchmarks/run_benchmark.py:251-255; tests/test_turboquant.py:
+ * 668-674). Here a search returns, per query, the k codes of smallest
+ * distance_m_to_n distance, ascending, ties broken by the smaller code index.
+ *
+ * A job is one query batch against one collection; jobs of one call may use
+ * different spaces (dims, bit widths) on the same device. All job buffers are
+ * device memory. */
+#define TQ_SEARCH_MAX_K 16
+
+typedef struct tq_search_job {
+  tq_space space;
+  const float *queries; /* [m, dim] float32 */
+  uint64_t m;
+  const uint8_t *codes; /* [n, code_size], n < 2^32 - 1 */
+  uint64_t n;
+  float *out_dist;      /* [m, k] float32, ascending */
+  int64_t *out_ids;     /* [m, k] row index into codes; -1 (dist +inf) past n */
+} tq_search_job;
+
+/* Knobs of the persistent launch (TQ_MODE_MFMA plans). 0 = automatic: the
+ * autotuner's table for the job's (padded dim, bits, m) bucket if it has an
+ * entry (tq_search_plan_tune), else the built-in rule. */
+typedef struct tq_search_tuning {
+  int32_t query_tile;   /* 32 or 64 queries per work item, or 0 */
+  int32_t items_per_wg; /* work items per persistent workgroup (load balance), or 0 */
+} tq_search_tuning;
+
+typedef struct tq_search_plan_s *tq_search_plan;
+
+/* Builds a reusable search over `jobs` (copied): checks every job, sizes the
+ * work queue and allocates all device scratch once. mode TQ_MODE_MFMA: one
+ * persistent launch over a (query tile, code range) queue with the top-k
+ * fused into the scan (k <= TQ_SEARCH_MAX_K). TQ_MODE_FP32: per job the plain
+ * fp32 distance matrix in row blocks, then a row top-k. `tuning` may be NULL. */
+int tq_search_plan_create(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
+                          const tq_search_tuning *tuning, tq_search_plan *out);
+/* Enqueues the plan on `stream`: query preparation, the scan and the top-k.
+ * No allocation and no host synchronisation, so it may be captured into a
+ * HIP graph; the job buffers are read when the work runs. */
+int tq_search_plan_run(tq_search_plan p, void *stream);
+/* As tq_search_plan_run, replaying a HIP graph of the plan's work (captured
+ * on the first call). */
+int tq_search_plan_run_graph(tq_search_plan p, void *stream);
+/* Times the candidate knobs for each (padded dim, bits, m) bucket of the
+ * plan's jobs on the plan's own buffers (overwriting the outputs), records
+ * the fastest in the process-wide table, and rebuilds the plan with them.
+ * Synchronous. `chosen` (may be NULL) receives the plan's resulting knobs. */
+int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen);
+/* Work items and persistent workgroups of the plan (either may be NULL). */
+int tq_search_plan_stats(tq_search_plan p, uint64_t *items, uint64_t *workgroups);
+int tq_search_plan_destroy(tq_search_plan p);
+
+/* One-shot: create, run on `stream`, wait, destroy. */
+int tq_search(const tq_search_job *jobs, uint64_t njobs, int k, int mode, void *stream);
+
+/* Row-wise top-k of a device float32 [m, n] matrix (ties by column index):
+ * out_dist / out_ids [m, k] on the device; m <= 65535, k <= TQ_SEARCH_MAX_K. */
+int tq_topk_rows(const float *dist, uint64_t m, uint64_t n, int k, float *out_dist,
+                 int64_t *out_ids, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

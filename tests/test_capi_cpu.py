@@ -54,3 +54,26 @@ def test_product_never_touches_the_oracle():
             if f.endswith((".py", ".cpp", ".hip", ".h")):
                 text = open(os.path.join(root, f)).read()
                 assert "oracle" not in text.lower(), os.path.join(root, f)
+
+
+def test_search_entry_points_validate_before_device_use():
+    from turboquant import capi
+    L = capi.lib()
+    h = ctypes.c_void_p()
+    assert L.tq_search_plan_create(None, 0, 10, capi.TQ_MODE_MFMA, None,
+                                   ctypes.byref(h)) == capi.TQ_EINVAL
+    assert b"no jobs" in L.tq_last_error()
+    jobs = capi.make_jobs([(0, 0, 1, 0, 1, 0, 0)])
+    assert L.tq_search_plan_create(jobs, 1, 10, capi.TQ_MODE_MFMA, None,
+                                   ctypes.byref(h)) == capi.TQ_EINVAL
+    assert b"no space" in L.tq_last_error()
+    assert L.tq_search_plan_create(jobs, 1, 0, capi.TQ_MODE_MFMA, None,
+                                   ctypes.byref(h)) == capi.TQ_EINVAL
+    assert L.tq_search_plan_create(jobs, 1, 10, 7, None, ctypes.byref(h)) == capi.TQ_EINVAL
+    bad = capi.SearchTuning(48, 0)
+    assert L.tq_search_plan_create(jobs, 1, 10, capi.TQ_MODE_MFMA, ctypes.byref(bad),
+                                   ctypes.byref(h)) == capi.TQ_EINVAL
+    assert L.tq_topk_rows(None, 1, 1, 17, None, None, None) == capi.TQ_EINVAL
+    assert L.tq_topk_rows(None, 70000, 1, 10, None, None, None) == capi.TQ_EINVAL
+    assert L.tq_search_plan_run(None, None) == capi.TQ_EINVAL
+    assert L.tq_search_plan_destroy(None) == capi.TQ_OK

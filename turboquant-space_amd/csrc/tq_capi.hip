@@ -12,13 +12,15 @@
 #include <string>
 #include <vector>
 
-#include "../../include/tq_hip.h"
-#include "tq_common.h"
+#include "tq_internal.h"
 #include "tq_mfma.h"
 
-namespace {
+namespace tq {
+namespace capi {
 
+namespace {
 thread_local std::string g_err;
+}
 
 int fail(int code, const char *fmt, ...) {
   char buf[512];
@@ -35,6 +37,24 @@ int hip_fail(hipError_t e, const char *what) {
     return fail(TQ_ENOMEM, "%s: out of device memory", what);
   return fail(TQ_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
+
+int cu_count(int device) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  return cus;
+}
+
+const char *last_error() { return g_err.c_str(); }
+
+}  // namespace capi
+}  // namespace tq
+
+namespace {
+
+using tq::capi::fail;
+using tq::capi::hip_fail;
 
 // ---- host tables ---------------------------------------------------------
 
@@ -107,17 +127,9 @@ struct DevBuf {
 
 }  // namespace
 
-struct tq_space_s {
-  tq_space_info info{};
-  int device = 0;
-  std::vector<float> cent, bnd, rot, qjl;
-  float *d_tables = nullptr;  // [cent L | bnd L-1 (+pad) | rot D | qjl D]
-  tq::SpaceDev dev{};
-};
-
 extern "C" {
 
-const char *tq_last_error(void) { return g_err.c_str(); }
+const char *tq_last_error(void) { return tq::capi::last_error(); }
 
 int tq_device_count(void) {
   int n = 0;

@@ -47,15 +47,18 @@ hipError_t launch_prepare(const SpaceDev &s, const float *Q, long m,
                           float *prep, hipStream_t st);
 hipError_t launch_prepare_from_codes(const SpaceDev &s, const uint8_t *codes,
                                      long m, float *prep, hipStream_t st);
-// Plain fp32 scan: out[q*ldo + j] for q < m, j < n.
+// Plain fp32 scan: out[q*n + j] for q < m, j < n. `ws`, if given, holds
+// scan_fp32_ws_bytes(s, m) bytes of scratch; without it the scan allocates
+// (stream-ordered) what it needs.
 hipError_t launch_scan_fp32(const SpaceDev &s, const float *prep, long m,
                             const uint8_t *codes, long n, float *out,
-                            bool symmetric, hipStream_t st);
+                            bool symmetric, hipStream_t st, void *ws = nullptr);
+size_t scan_fp32_ws_bytes(const SpaceDev &s, long m);
 // 4-bit codes, 32 <= D <= 1024: pair-table scan (tq_lut.hip).
 bool lut_supported(const SpaceDev &s);
 hipError_t launch_scan_lut(const SpaceDev &s, const float *prep, long m,
                            const uint8_t *codes, long n, float *out,
-                           bool symmetric, hipStream_t st);
+                           bool symmetric, hipStream_t st, void *ws = nullptr);
 bool encode_supported(int D);
 bool scan_supported(int D);
 

@@ -299,16 +299,26 @@ hipError_t launch_lut(const SpaceDev &s, const float *prep, const int32_t *ql, l
 
 }  // namespace
 
+size_t scan_fp32_ws_bytes(const SpaceDev &s, long m) {
+  return lut_supported(s) ? sizeof(int32_t) * (size_t)m * qlimb_stride(s.D) : 0;
+}
+
 hipError_t launch_scan_lut(const SpaceDev &s, const float *prep, long m, const uint8_t *codes,
-                           long n, float *out, bool sym, hipStream_t st) {
+                           long n, float *out, bool sym, hipStream_t st, void *ws) {
   if (!lut_supported(s)) return hipErrorNotSupported;
   if (m <= 0 || n <= 0) return hipSuccess;
   int32_t *ql = nullptr;
+  bool owned = false;
   hipError_t e = hipSuccess;
   if (!sym) {
-    e = hipMallocAsync(reinterpret_cast<void **>(&ql),
-                       sizeof(int32_t) * (size_t)m * qlimb_stride(s.D), st);
-    if (e != hipSuccess) return e;
+    if (ws) {
+      ql = static_cast<int32_t *>(ws);
+    } else {
+      e = hipMallocAsync(reinterpret_cast<void **>(&ql),
+                         sizeof(int32_t) * (size_t)m * qlimb_stride(s.D), st);
+      if (e != hipSuccess) return e;
+      owned = true;
+    }
     const long g = m < 65535 ? m : 65535;
     hipLaunchKernelGGL(k_qjl_limbs, dim3((unsigned)g), dim3(256), 0, st, s, prep, m, ql);
     e = hipGetLastError();
@@ -316,7 +326,7 @@ hipError_t launch_scan_lut(const SpaceDev &s, const float *prep, long m, const u
   if (e == hipSuccess)
     e = sym ? launch_lut<true>(s, prep, ql, m, codes, n, out, st)
             : launch_lut<false>(s, prep, ql, m, codes, n, out, st);
-  if (ql) {
+  if (owned) {
     hipError_t f = hipFreeAsync(ql, st);
     if (e == hipSuccess) e = f;
   }

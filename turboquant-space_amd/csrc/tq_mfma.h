@@ -9,4 +9,8 @@ namespace tq {
 hipError_t launch_scan_mfma(const SpaceDev &s, const float *prep, long m,
                             const uint8_t *codes, long n, float *out,
                             hipStream_t st);
+// Query hi/lo planes [3][m][D] fp16 for the MFMA kernels (4-bit spaces store
+// each 8-coordinate group in the nibble decoder's order).
+hipError_t launch_split_queries(const SpaceDev &s, const float *prep, long m,
+                                _Float16 *planes, hipStream_t st);
 }  // namespace tq

@@ -1,0 +1,689 @@
+// tq_plan.hip -- host side of the top-k search (include/tq_hip.h): search
+// plans, their device workspace, the work queue, HIP graph capture and the
+// autotuner table.
+//
+// A plan coalesces its jobs by collection (same space, codes and n): the
+// queries of all jobs aimed at one collection are gathered into one batch,
+// prepared and split once, and scored in one sweep over the collection's
+// codes. Results land in plan-owned [rows, k] buffers and are scattered back
+// to each job's outputs. Every step is a kernel launch on the caller's
+// stream, so a plan replays as a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <tuple>
+#include <vector>
+
+#include "tq_internal.h"
+#include "tq_mfma.h"
+#include "tq_search.h"
+
+namespace tq {
+namespace {
+
+using capi::fail;
+using capi::hip_fail;
+
+// Contiguous float copies (query gather) and top-k result copies (scatter).
+struct CopySeg {
+  const float *src;
+  float *dst;
+  long count;  // floats
+};
+struct OutSeg {
+  const float *src_d;
+  const int64_t *src_i;
+  float *dst_d;
+  int64_t *dst_i;
+  long count;  // entries
+};
+
+__global__ void k_gather(const CopySeg *__restrict__ segs) {
+  const CopySeg sg = segs[blockIdx.y];
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < sg.count;
+       i += (long)gridDim.x * blockDim.x)
+    sg.dst[i] = sg.src[i];
+}
+
+__global__ void k_scatter(const OutSeg *__restrict__ segs) {
+  const OutSeg sg = segs[blockIdx.y];
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < sg.count;
+       i += (long)gridDim.x * blockDim.x) {
+    sg.dst_d[i] = sg.src_d[i];
+    sg.dst_i[i] = sg.src_i[i];
+  }
+}
+
+// ---- autotuner table: (padded dim, bits, m bucket) -> knobs --------------
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, tq_search_tuning> g_tune;
+
+int m_bucket(long m) {
+  int b = 0;
+  while ((1L << b) < m) ++b;
+  return b;  // ceil(log2 m): 1 -> 0, 2 -> 1, 3..4 -> 2, ...
+}
+
+bool tune_lookup(int D, int bits, long m, tq_search_tuning *out) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find({D, bits, m_bucket(m)});
+  if (it == g_tune.end()) return false;
+  *out = it->second;
+  return true;
+}
+
+void tune_store(int D, int bits, long m, const tq_search_tuning &t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[{D, bits, m_bucket(m)}] = t;
+}
+
+constexpr int kDefaultItemsPerWg = 4;
+constexpr size_t kFp32BlockBytes = 512ull << 20;  // distance rows per block (FP32 plans)
+
+size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+}  // namespace
+}  // namespace tq
+
+// One collection and the jobs aimed at it.
+struct PlanBatch {
+  tq_space space;
+  const uint8_t *codes;
+  long n;
+  long m = 0;                 // rows over all its jobs
+  long row0 = 0;              // first row in the plan-owned outputs
+  std::vector<int> jobs;      // indices into plan->jobs, in row order
+  std::vector<long> job_row;  // first row of each job within the batch
+  int qr = 1;                 // query tile: 32 * qr
+  // device pointers inside the workspace
+  float *queries = nullptr, *prep = nullptr;
+  _Float16 *planes = nullptr;
+};
+
+struct tq_search_plan_s {
+  int device = 0;
+  int k = 0;
+  int mode = TQ_MODE_MFMA;
+  int cus = 256;
+  tq_search_tuning forced{};  // knobs the caller fixed (0 = automatic)
+  tq_search_tuning used{};    // knobs of the current build
+  std::vector<tq_search_job> jobs;
+  std::vector<PlanBatch> batches;
+  long rows = 0;
+
+  // host tables (uploaded at build)
+  std::vector<tq::SearchJobDev> h_jobs;
+  std::vector<tq::SearchItem> h_items;
+  std::vector<int> h_order;
+  std::vector<tq::SearchGroup> h_groups;
+  std::vector<tq::CopySeg> h_gather;
+  std::vector<tq::OutSeg> h_scatter;
+  int nitems_qr[2] = {0, 0};
+  int grid_qr[2] = {0, 0};
+  long gather_max = 0, scatter_max = 0;
+
+  // device workspace (one allocation)
+  void *ws = nullptr;
+  size_t ws_bytes = 0;
+  tq::SearchJobDev *d_jobs = nullptr;
+  tq::SearchItem *d_items = nullptr;
+  int *d_order = nullptr;
+  tq::SearchGroup *d_groups = nullptr;
+  tq::CopySeg *d_gather = nullptr;
+  tq::OutSeg *d_scatter = nullptr;
+  int *d_counter = nullptr;
+  float *cand_d = nullptr;
+  uint32_t *cand_i = nullptr;
+  float *all_d = nullptr;
+  int64_t *all_i = nullptr;
+  // FP32 plans
+  float *dist = nullptr;
+  float *seg_d = nullptr;
+  uint32_t *seg_i = nullptr;
+  void *scan_ws = nullptr;
+
+  // graph
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  void drop_ws() {
+    if (ws) (void)hipFree(ws);
+    ws = nullptr;
+    ws_bytes = 0;
+  }
+  ~tq_search_plan_s() {
+    drop_graph();
+    drop_ws();
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+};
+
+namespace {
+
+using tq::capi::fail;
+using tq::capi::hip_fail;
+
+long fp32_block_rows(const PlanBatch &b) {
+  long r = (long)(tq::kFp32BlockBytes / (sizeof(float) * (size_t)std::max(b.n, 1L)));
+  r = std::max(1L, std::min(r, b.m));
+  return std::min(r, 65535L);
+}
+
+// Knobs for one batch: forced > tuned table > built-in rule.
+tq_search_tuning knobs_for(const tq_search_plan_s *p, const PlanBatch &b) {
+  tq_search_tuning t{};
+  const tq_space_info &info = b.space->info;
+  tq_search_tuning tab{};
+  const bool have = tq::tune_lookup((int)info.padded_dim, info.bits_per_coord, b.m, &tab);
+  t.query_tile = p->forced.query_tile ? p->forced.query_tile
+                 : have               ? tab.query_tile
+                                      : (b.m <= 32 ? 32 : 64);
+  t.items_per_wg = p->forced.items_per_wg ? p->forced.items_per_wg
+                   : have                 ? tab.items_per_wg
+                                          : tq::kDefaultItemsPerWg;
+  return t;
+}
+
+// Work queue, groups and copy tables for the current knobs; then the
+// workspace (grown if needed) and the upload of the tables.
+int build(tq_search_plan_s *p) {
+  const char *what = "tq_search_plan";
+  p->drop_graph();
+  p->h_jobs.clear();
+  p->h_items.clear();
+  p->h_order.clear();
+  p->h_groups.clear();
+  p->h_gather.clear();
+  p->h_scatter.clear();
+  p->nitems_qr[0] = p->nitems_qr[1] = 0;
+  p->grid_qr[0] = p->grid_qr[1] = 0;
+  p->gather_max = p->scatter_max = 0;
+
+  // Plan-level items per workgroup: the knob of the costliest batch.
+  int ipw = tq::kDefaultItemsPerWg;
+  double worst = -1.0;
+  for (PlanBatch &b : p->batches) {
+    const tq_search_tuning t = knobs_for(p, b);
+    b.qr = t.query_tile == 64 ? 2 : 1;
+    const double cost = (double)b.m * (double)b.n * (double)b.space->info.padded_dim;
+    if (cost > worst) {
+      worst = cost;
+      ipw = std::max(1, (int)t.items_per_wg);
+    }
+  }
+  p->used.items_per_wg = ipw;
+  p->used.query_tile = 0;
+
+  // Items per query-tile kind, sized so each kind's queue holds about
+  // ipw items per persistent workgroup; within a kind the largest go first.
+  const int ntile = tq::kSearchNTile;
+  struct Pending {
+    tq::SearchItem it;
+    double cost;
+    int kind;
+  };
+  std::vector<Pending> pend;
+  if (p->mode == TQ_MODE_MFMA) {
+    for (int kind = 1; kind <= 2; ++kind) {
+      double total = 0.0;
+      for (const PlanBatch &b : p->batches) {
+        if (b.qr != kind || b.m == 0) continue;
+        const long qtiles = (b.m + 32 * kind - 1) / (32 * kind);
+        const long ntiles = (b.n + ntile - 1) / ntile;
+        total += (double)qtiles * ntiles * b.space->info.padded_dim * kind;
+      }
+      const long wg = (long)p->cus * (kind == 1 ? 2 : 1);
+      const double target = total / (double)(wg * ipw);
+      for (size_t bi = 0; bi < p->batches.size(); ++bi) {
+        const PlanBatch &b = p->batches[bi];
+        if (b.qr != kind) continue;
+        const long qtiles = (b.m + 32 * kind - 1) / (32 * kind);
+        const long ntiles = (b.n + ntile - 1) / ntile;
+        const double tile_cost = (double)b.space->info.padded_dim * kind;
+        long tpi = (long)std::llround(target / tile_cost);
+        tpi = std::max(1L, std::min(tpi, std::max(ntiles, 1L)));
+        for (long qt = 0; qt < qtiles; ++qt) {
+          tq::SearchGroup g{(int)bi, (int)qt, 0, 0, kind};
+          g.ibeg = (int)pend.size();
+          for (long t0 = 0; t0 < ntiles; t0 += tpi) {
+            const long t1 = std::min(ntiles, t0 + tpi);
+            pend.push_back({{(int)bi, (int)qt, (int)t0, (int)t1}, (double)(t1 - t0) * tile_cost,
+                            kind});
+          }
+          g.iend = (int)pend.size();
+          p->h_groups.push_back(g);
+        }
+      }
+    }
+    if (pend.size() > (size_t)INT32_MAX / 4)
+      return fail(TQ_EINVAL, "%s: too many work items (%zu)", what, pend.size());
+    p->h_items.reserve(pend.size());
+    for (const Pending &pi : pend) p->h_items.push_back(pi.it);
+    std::vector<int> ord(pend.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+      if (pend[a].kind != pend[b].kind) return pend[a].kind < pend[b].kind;
+      return pend[a].cost > pend[b].cost;
+    });
+    p->h_order = ord;
+    for (const Pending &pi : pend) p->nitems_qr[pi.kind - 1]++;
+    for (int kind = 1; kind <= 2; ++kind) {
+      const long wg = (long)p->cus * (kind == 1 ? 2 : 1);
+      p->grid_qr[kind - 1] = (int)std::min<long>(wg, p->nitems_qr[kind - 1]);
+    }
+  }
+
+  // Workspace layout.
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += tq::align_up(bytes ? bytes : 4);
+    return at;
+  };
+  const size_t nb = p->batches.size();
+  const size_t o_jobs = take(nb * sizeof(tq::SearchJobDev));
+  const size_t o_items = take(p->h_items.size() * sizeof(tq::SearchItem));
+  const size_t o_order = take(p->h_order.size() * sizeof(int));
+  const size_t o_groups = take(p->h_groups.size() * sizeof(tq::SearchGroup));
+  const size_t o_gather = take(p->jobs.size() * sizeof(tq::CopySeg));
+  const size_t o_scatter = take(p->jobs.size() * sizeof(tq::OutSeg));
+  const size_t o_counter = take(2 * sizeof(int));
+  const size_t cand = p->h_items.size() * 4 * 64 * (size_t)tq::kSearchMaxK;
+  const size_t o_cand_d = take(cand * sizeof(float));
+  const size_t o_cand_i = take(cand * sizeof(uint32_t));
+  const size_t o_all_d = take((size_t)p->rows * p->k * sizeof(float));
+  const size_t o_all_i = take((size_t)p->rows * p->k * sizeof(int64_t));
+  std::vector<size_t> o_q(nb), o_prep(nb), o_planes(nb);
+  size_t dist_elems = 0, seg_elems = 0, scan_ws = 0;
+  for (size_t bi = 0; bi < nb; ++bi) {
+    const PlanBatch &b = p->batches[bi];
+    const tq_space_info &info = b.space->info;
+    o_q[bi] = take((size_t)b.m * info.dim * sizeof(float));
+    o_prep[bi] = take((size_t)b.m * info.prepared_stride * sizeof(float));
+    if (p->mode == TQ_MODE_MFMA) {
+      o_planes[bi] = take(3 * (size_t)b.m * info.padded_dim * sizeof(_Float16));
+    } else if (b.m > 0) {
+      const long rows = fp32_block_rows(b);
+      dist_elems = std::max(dist_elems, (size_t)rows * b.n);
+      // Any block of r <= rows rows uses r * topk_rows_segments(r, n) < 2048 + rows lists.
+      seg_elems = std::max(seg_elems, (size_t)(2048 + rows) * tq::kSearchMaxK);
+      scan_ws = std::max(scan_ws, tq::scan_fp32_ws_bytes(b.space->dev, rows));
+    }
+  }
+  const size_t o_dist = take(dist_elems * sizeof(float));
+  const size_t o_seg_d = take(seg_elems * sizeof(float));
+  const size_t o_seg_i = take(seg_elems * sizeof(uint32_t));
+  const size_t o_scan_ws = take(scan_ws);
+
+  hipError_t e = hipSetDevice(p->device);
+  if (e != hipSuccess) return hip_fail(e, what);
+  if (off > p->ws_bytes) {
+    p->drop_ws();
+    e = hipMalloc(&p->ws, off);
+    if (e != hipSuccess) return hip_fail(e, what);
+    p->ws_bytes = off;
+  }
+  char *w = static_cast<char *>(p->ws);
+  p->d_jobs = reinterpret_cast<tq::SearchJobDev *>(w + o_jobs);
+  p->d_items = reinterpret_cast<tq::SearchItem *>(w + o_items);
+  p->d_order = reinterpret_cast<int *>(w + o_order);
+  p->d_groups = reinterpret_cast<tq::SearchGroup *>(w + o_groups);
+  p->d_gather = reinterpret_cast<tq::CopySeg *>(w + o_gather);
+  p->d_scatter = reinterpret_cast<tq::OutSeg *>(w + o_scatter);
+  p->d_counter = reinterpret_cast<int *>(w + o_counter);
+  p->cand_d = reinterpret_cast<float *>(w + o_cand_d);
+  p->cand_i = reinterpret_cast<uint32_t *>(w + o_cand_i);
+  p->all_d = reinterpret_cast<float *>(w + o_all_d);
+  p->all_i = reinterpret_cast<int64_t *>(w + o_all_i);
+  p->dist = reinterpret_cast<float *>(w + o_dist);
+  p->seg_d = reinterpret_cast<float *>(w + o_seg_d);
+  p->seg_i = reinterpret_cast<uint32_t *>(w + o_seg_i);
+  p->scan_ws = scan_ws ? static_cast<void *>(w + o_scan_ws) : nullptr;
+
+  for (size_t bi = 0; bi < nb; ++bi) {
+    PlanBatch &b = p->batches[bi];
+    b.queries = reinterpret_cast<float *>(w + o_q[bi]);
+    b.prep = reinterpret_cast<float *>(w + o_prep[bi]);
+    b.planes = p->mode == TQ_MODE_MFMA ? reinterpret_cast<_Float16 *>(w + o_planes[bi]) : nullptr;
+    tq::SearchJobDev jd{};
+    jd.s = b.space->dev;
+    jd.planes = b.planes;
+    jd.prep = b.prep;
+    jd.codes = b.codes;
+    jd.m = b.m;
+    jd.n = b.n;
+    jd.out_d = p->all_d + b.row0 * p->k;
+    jd.out_i = p->all_i + b.row0 * p->k;
+    p->h_jobs.push_back(jd);
+    const long dim = (long)b.space->info.dim;
+    for (size_t t = 0; t < b.jobs.size(); ++t) {
+      const tq_search_job &j = p->jobs[b.jobs[t]];
+      if (j.m == 0) continue;
+      const long r0 = b.row0 + b.job_row[t];
+      p->h_gather.push_back({j.queries, b.queries + b.job_row[t] * dim, (long)j.m * dim});
+      p->h_scatter.push_back({p->all_d + r0 * p->k, p->all_i + r0 * p->k, j.out_dist, j.out_ids,
+                              (long)j.m * p->k});
+      p->gather_max = std::max(p->gather_max, (long)j.m * dim);
+      p->scatter_max = std::max(p->scatter_max, (long)j.m * p->k);
+    }
+  }
+  auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
+    return bytes ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+  };
+  if ((e = up(p->d_jobs, p->h_jobs.data(), nb * sizeof(tq::SearchJobDev))) != hipSuccess ||
+      (e = up(p->d_items, p->h_items.data(), p->h_items.size() * sizeof(tq::SearchItem))) !=
+          hipSuccess ||
+      (e = up(p->d_order, p->h_order.data(), p->h_order.size() * sizeof(int))) != hipSuccess ||
+      (e = up(p->d_groups, p->h_groups.data(), p->h_groups.size() * sizeof(tq::SearchGroup))) !=
+          hipSuccess ||
+      (e = up(p->d_gather, p->h_gather.data(), p->h_gather.size() * sizeof(tq::CopySeg))) !=
+          hipSuccess ||
+      (e = up(p->d_scatter, p->h_scatter.data(), p->h_scatter.size() * sizeof(tq::OutSeg))) !=
+          hipSuccess)
+    return hip_fail(e, what);
+  return TQ_OK;
+}
+
+unsigned copy_blocks(long max_count) {
+  long b = (max_count + 255) / 256;
+  return (unsigned)std::max(1L, std::min(b, 1024L));
+}
+
+// Everything the plan does, enqueued on `st`.
+int enqueue(tq_search_plan_s *p, hipStream_t st) {
+  const char *what = "tq_search_plan_run";
+  hipError_t e;
+  if (!p->h_gather.empty()) {
+    hipLaunchKernelGGL(tq::k_gather, dim3(copy_blocks(p->gather_max), (unsigned)p->h_gather.size()),
+                       dim3(256), 0, st, p->d_gather);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, what);
+  }
+  for (const PlanBatch &b : p->batches) {
+    if (b.m == 0) continue;
+    if ((e = tq::launch_prepare(b.space->dev, b.queries, b.m, b.prep, st)) != hipSuccess)
+      return hip_fail(e, what);
+    if (p->mode == TQ_MODE_MFMA &&
+        (e = tq::launch_split_queries(b.space->dev, b.prep, b.m, b.planes, st)) != hipSuccess)
+      return hip_fail(e, what);
+  }
+  if (p->mode == TQ_MODE_MFMA) {
+    e = tq::launch_search(p->d_jobs, p->d_items, p->d_order, p->nitems_qr, p->d_groups,
+                          (int)p->h_groups.size(), p->k, p->d_counter, p->cand_d, p->cand_i,
+                          p->grid_qr, st);
+    if (e != hipSuccess) return hip_fail(e, what);
+  } else {
+    for (const PlanBatch &b : p->batches) {
+      if (b.m == 0) continue;
+      float *od = p->all_d + b.row0 * p->k;
+      int64_t *oi = p->all_i + b.row0 * p->k;
+      // With no codes the scan is skipped and the top-k writes empty entries.
+      const long rows = fp32_block_rows(b);
+      const long stride = (long)b.space->info.prepared_stride;
+      for (long r0 = 0; r0 < b.m; r0 += rows) {
+        const long rc = std::min(rows, b.m - r0);
+        e = tq::launch_scan_fp32(b.space->dev, b.prep + r0 * stride, rc, b.codes, b.n, p->dist,
+                                 false, st, p->scan_ws);
+        if (e != hipSuccess) return hip_fail(e, what);
+        e = tq::launch_topk_rows(p->dist, rc, b.n, p->k, od + r0 * p->k, oi + r0 * p->k,
+                                 p->seg_d, p->seg_i, st);
+        if (e != hipSuccess) return hip_fail(e, what);
+      }
+    }
+  }
+  if (!p->h_scatter.empty()) {
+    hipLaunchKernelGGL(tq::k_scatter,
+                       dim3(copy_blocks(p->scatter_max), (unsigned)p->h_scatter.size()), dim3(256),
+                       0, st, p->d_scatter);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, what);
+  }
+  return TQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tq_search_plan_create(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
+                          const tq_search_tuning *tuning, tq_search_plan *out) {
+  const char *what = "tq_search_plan_create";
+  if (!out) return fail(TQ_EINVAL, "%s: out is NULL", what);
+  *out = nullptr;
+  if (!jobs || njobs == 0) return fail(TQ_EINVAL, "%s: no jobs", what);
+  if (njobs > 65535) return fail(TQ_EINVAL, "%s: at most 65535 jobs per plan", what);
+  if (k < 1 || k > TQ_SEARCH_MAX_K)
+    return fail(TQ_EINVAL, "%s: k must be in [1, %d], got %d", what, TQ_SEARCH_MAX_K, k);
+  if (mode != TQ_MODE_MFMA && mode != TQ_MODE_FP32)
+    return fail(TQ_EINVAL, "%s: unknown mode %d", what, mode);
+  if (tuning) {
+    if (tuning->query_tile != 0 && tuning->query_tile != 32 && tuning->query_tile != 64)
+      return fail(TQ_EINVAL, "%s: query_tile must be 0, 32 or 64", what);
+    if (tuning->items_per_wg < 0 || tuning->items_per_wg > 1024)
+      return fail(TQ_EINVAL, "%s: items_per_wg must be in [0, 1024]", what);
+  }
+  int device = -1;
+  for (uint64_t i = 0; i < njobs; ++i) {
+    const tq_search_job &j = jobs[i];
+    if (!j.space) return fail(TQ_EINVAL, "%s: job %llu has no space", what, (unsigned long long)i);
+    if (device < 0) device = j.space->device;
+    if (j.space->device != device)
+      return fail(TQ_EINVAL, "%s: job %llu is on device %d, job 0 on %d", what,
+                  (unsigned long long)i, j.space->device, device);
+    if (j.n >= 0xFFFFFFFFull)
+      return fail(TQ_EINVAL, "%s: job %llu: n must be below 2^32 - 1", what, (unsigned long long)i);
+    if (j.m > (1ull << 31))
+      return fail(TQ_EINVAL, "%s: job %llu: m must be at most 2^31", what, (unsigned long long)i);
+    if (j.m > 0 && (!j.queries || !j.out_dist || !j.out_ids || (j.n > 0 && !j.codes)))
+      return fail(TQ_EINVAL, "%s: job %llu has a NULL buffer", what, (unsigned long long)i);
+    if (mode == TQ_MODE_MFMA && j.space->dev.D < 32)
+      return fail(TQ_EUNSUPPORTED, "%s: job %llu: MFMA search needs padded dim >= 32, got %d", what,
+                  (unsigned long long)i, j.space->dev.D);
+  }
+  tq_search_plan_s *p = new (std::nothrow) tq_search_plan_s();
+  if (!p) return fail(TQ_ENOMEM, "%s: host allocation failed", what);
+  p->device = device;
+  p->k = k;
+  p->mode = mode;
+  p->cus = tq::capi::cu_count(device);
+  if (tuning) p->forced = *tuning;
+  p->jobs.assign(jobs, jobs + njobs);
+  // Coalesce by collection, in first-seen order.
+  for (uint64_t i = 0; i < njobs; ++i) {
+    const tq_search_job &j = jobs[i];
+    PlanBatch *b = nullptr;
+    for (PlanBatch &c : p->batches)
+      if (c.space == j.space && c.codes == j.codes && c.n == (long)j.n) b = &c;
+    if (!b) {
+      p->batches.push_back(PlanBatch{});
+      b = &p->batches.back();
+      b->space = j.space;
+      b->codes = j.codes;
+      b->n = (long)j.n;
+    }
+    b->jobs.push_back((int)i);
+    b->job_row.push_back(b->m);
+    b->m += (long)j.m;
+  }
+  for (PlanBatch &b : p->batches) {
+    b.row0 = p->rows;
+    p->rows += b.m;
+    if (mode == TQ_MODE_MFMA && b.m > (1L << 31) - 64)
+      return delete p, fail(TQ_EINVAL, "%s: too many queries for one collection", what);
+  }
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return delete p, hip_fail(e, what);
+  if (int r = build(p)) return delete p, r;
+  *out = p;
+  return TQ_OK;
+}
+
+int tq_search_plan_run(tq_search_plan p, void *stream) {
+  if (!p) return fail(TQ_EINVAL, "tq_search_plan_run: NULL plan");
+  hipError_t e = hipSetDevice(p->device);
+  if (e != hipSuccess) return hip_fail(e, "tq_search_plan_run");
+  return enqueue(p, static_cast<hipStream_t>(stream));
+}
+
+int tq_search_plan_run_graph(tq_search_plan p, void *stream) {
+  const char *what = "tq_search_plan_run_graph";
+  if (!p) return fail(TQ_EINVAL, "%s: NULL plan", what);
+  hipError_t e = hipSetDevice(p->device);
+  if (e != hipSuccess) return hip_fail(e, what);
+  if (!p->exec) {
+    if (!p->cap && (e = hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking)) != hipSuccess)
+      return hip_fail(e, what);
+    if ((e = hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return hip_fail(e, what);
+    const int r = enqueue(p, p->cap);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(p->cap, &g);
+    if (r) {
+      if (g) (void)hipGraphDestroy(g);
+      return r;
+    }
+    if (e != hipSuccess) return hip_fail(e, what);
+    p->graph = g;
+    if ((e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0)) != hipSuccess) {
+      p->drop_graph();
+      return hip_fail(e, what);
+    }
+  }
+  e = hipGraphLaunch(p->exec, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, what);
+  return TQ_OK;
+}
+
+int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen) {
+  const char *what = "tq_search_plan_tune";
+  if (!p) return fail(TQ_EINVAL, "%s: NULL plan", what);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSetDevice(p->device);
+  if (e != hipSuccess) return hip_fail(e, what);
+  if (p->mode == TQ_MODE_MFMA) {
+    hipEvent_t t0, t1;
+    if ((e = hipEventCreate(&t0)) != hipSuccess) return hip_fail(e, what);
+    if ((e = hipEventCreate(&t1)) != hipSuccess) {
+      (void)hipEventDestroy(t0);
+      return hip_fail(e, what);
+    }
+    const tq_search_tuning saved = p->forced;
+    int rc = TQ_OK;
+    // Each collection batch alone: a sub-plan over that batch's jobs.
+    for (size_t bi = 0; bi < p->batches.size() && rc == TQ_OK; ++bi) {
+      const PlanBatch &b = p->batches[bi];
+      if (b.m == 0 || b.n == 0) continue;
+      std::vector<tq_search_job> sub;
+      for (int ji : b.jobs) sub.push_back(p->jobs[ji]);
+      tq_search_tuning best{};
+      float best_ms = 0.0f;
+      const int tiles[2] = {32, 64};
+      const int ipws[4] = {1, 2, 4, 8};
+      for (int qt : tiles) {
+        for (int ipw : ipws) {
+          tq_search_tuning cand{qt, ipw};
+          tq_search_plan sp = nullptr;
+          rc = tq_search_plan_create(sub.data(), sub.size(), p->k, p->mode, &cand, &sp);
+          if (rc) break;
+          float ms = 0.0f, run_best = 0.0f;
+          for (int rep = 0; rep < 4 && rc == TQ_OK; ++rep) {
+            if ((e = hipEventRecord(t0, st)) != hipSuccess) rc = hip_fail(e, what);
+            if (!rc) rc = enqueue(sp, st);
+            if (!rc && (e = hipEventRecord(t1, st)) != hipSuccess) rc = hip_fail(e, what);
+            if (!rc && (e = hipEventSynchronize(t1)) != hipSuccess) rc = hip_fail(e, what);
+            if (!rc && (e = hipEventElapsedTime(&ms, t0, t1)) != hipSuccess) rc = hip_fail(e, what);
+            if (!rc && rep > 0 && (rep == 1 || ms < run_best)) run_best = ms;  // rep 0 warms up
+          }
+          tq_search_plan_destroy(sp);
+          if (rc) break;
+          if (best.query_tile == 0 || run_best < best_ms) {
+            best = cand;
+            best_ms = run_best;
+          }
+        }
+        if (rc) break;
+      }
+      if (rc == TQ_OK)
+        tq::tune_store((int)b.space->info.padded_dim, b.space->info.bits_per_coord, b.m, best);
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc) return rc;
+    p->forced = saved;
+    if (int r = build(p)) return r;
+  }
+  if (chosen) {
+    *chosen = p->used;
+    // The query tile of the costliest batch.
+    double worst = -1.0;
+    for (const PlanBatch &b : p->batches) {
+      const double cost = (double)b.m * (double)b.n * (double)b.space->info.padded_dim;
+      if (cost > worst) {
+        worst = cost;
+        chosen->query_tile = 32 * b.qr;
+      }
+    }
+  }
+  return TQ_OK;
+}
+
+int tq_search_plan_stats(tq_search_plan p, uint64_t *items, uint64_t *workgroups) {
+  if (!p) return fail(TQ_EINVAL, "tq_search_plan_stats: NULL plan");
+  if (items) *items = p->h_items.size();
+  if (workgroups) *workgroups = (uint64_t)p->grid_qr[0] + (uint64_t)p->grid_qr[1];
+  return TQ_OK;
+}
+
+int tq_search_plan_destroy(tq_search_plan p) {
+  delete p;
+  return TQ_OK;
+}
+
+int tq_search(const tq_search_job *jobs, uint64_t njobs, int k, int mode, void *stream) {
+  tq_search_plan p = nullptr;
+  if (int r = tq_search_plan_create(jobs, njobs, k, mode, nullptr, &p)) return r;
+  int r = tq_search_plan_run(p, stream);
+  if (r == TQ_OK) {
+    hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) r = hip_fail(e, "tq_search");
+  }
+  tq_search_plan_destroy(p);
+  return r;
+}
+
+int tq_topk_rows(const float *dist, uint64_t m, uint64_t n, int k, float *out_dist,
+                 int64_t *out_ids, void *stream) {
+  const char *what = "tq_topk_rows";
+  if (k < 1 || k > TQ_SEARCH_MAX_K)
+    return fail(TQ_EINVAL, "%s: k must be in [1, %d], got %d", what, TQ_SEARCH_MAX_K, k);
+  if (m > 65535) return fail(TQ_EINVAL, "%s: m must be at most 65535", what);
+  if (n >= 0xFFFFFFFFull) return fail(TQ_EINVAL, "%s: n must be below 2^32 - 1", what);
+  if (m == 0) return TQ_OK;
+  if (!out_dist || !out_ids || (n > 0 && !dist)) return fail(TQ_EINVAL, "%s: NULL buffer", what);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const size_t seg = (size_t)m * tq::topk_rows_segments((long)m, (long)n) * tq::kSearchMaxK;
+  void *buf = nullptr;
+  hipError_t e = hipMallocAsync(&buf, tq::align_up(seg * sizeof(float)) + seg * sizeof(uint32_t), st);
+  if (e != hipSuccess) return hip_fail(e, what);
+  float *sd = static_cast<float *>(buf);
+  uint32_t *si = reinterpret_cast<uint32_t *>(static_cast<char *>(buf) +
+                                              tq::align_up(seg * sizeof(float)));
+  e = tq::launch_topk_rows(dist, (long)m, (long)n, k, out_dist, out_ids, sd, si, st);
+  hipError_t f = hipFreeAsync(buf, st);
+  if (e != hipSuccess) return hip_fail(e, what);
+  if (f != hipSuccess) return hip_fail(f, what);
+  return TQ_OK;
+}
+
+}  // extern "C"

@@ -173,9 +173,9 @@ hipError_t launch_cpl(const SpaceDev &s, const float *prep, long m, const uint8_
 bool scan_supported(int D) { return D >= 4 && D <= 4096; }
 
 hipError_t launch_scan_fp32(const SpaceDev &s, const float *prep, long m, const uint8_t *codes,
-                            long n, float *out, bool symmetric, hipStream_t st) {
+                            long n, float *out, bool symmetric, hipStream_t st, void *ws) {
   if (m <= 0 || n <= 0) return hipSuccess;
-  if (lut_supported(s)) return launch_scan_lut(s, prep, m, codes, n, out, symmetric, st);
+  if (lut_supported(s)) return launch_scan_lut(s, prep, m, codes, n, out, symmetric, st, ws);
   switch (s.D) {
     case 4: return launch_cpl<4>(s, prep, m, codes, n, out, symmetric, st);
     case 8: return launch_cpl<8>(s, prep, m, codes, n, out, symmetric, st);

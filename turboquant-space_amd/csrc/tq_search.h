@@ -1,0 +1,51 @@
+// tq_search.h -- persistent top-k search over a queue of work items.
+#pragma once
+#include "tq_common.h"
+
+namespace tq {
+
+constexpr int kSearchMaxK = 16;     // per-list capacity; callers ask for k <= this
+constexpr int kSearchNTile = 256;   // codes per tile
+
+// One (collection, query batch) pair: prepared queries, their fp16 planes
+// (launch_split_queries), the codes, and where the top-k goes.
+struct SearchJobDev {
+  SpaceDev s;
+  const _Float16 *planes;  // [3][m][D]
+  const float *prep;       // [m][prep_stride(D)]
+  const uint8_t *codes;    // [n][cs]
+  long m, n;
+  float *out_d;            // [m][k]
+  int64_t *out_i;          // [m][k]; -1 where fewer than k codes exist
+};
+
+// Code tiles [t0, t1) of query tile `qtile` (of 32 * qr queries) of job `job`.
+struct SearchItem {
+  int job, qtile, t0, t1;
+};
+
+// The items [ibeg, iend) that cover query tile `qtile` (of 32 * qr queries)
+// of job `job`; all items of a group share its qr.
+struct SearchGroup {
+  int job, qtile, ibeg, iend, qr;
+};
+
+int search_max_k();
+// Per query-tile kind (qr = 1: 32 queries, qr = 2: 64 queries): resets that
+// kind's queue counter (d_counter[qr - 1]) and runs a persistent search over
+// grid_qr[qr - 1] workgroups on the nitems_qr[qr - 1] items listed next in
+// `order` (all qr = 1 items first); then merges each query's candidate lists
+// into its top-k. Candidate buffers hold nitems * 4 * 64 * kSearchMaxK
+// entries, indexed by item.
+hipError_t launch_search(const SearchJobDev *d_jobs, const SearchItem *d_items,
+                         const int *d_order, const int *nitems_qr, const SearchGroup *d_groups,
+                         int ngroups, int k, int *d_counter, float *d_cand_d,
+                         uint32_t *d_cand_i, const int *grid_qr, hipStream_t st);
+// Row-wise top-k of a dense [m][n] distance matrix (ties by column index),
+// m <= 65535, n < 2^32 - 1. Scratch: seg_d / seg_i of
+// m * topk_rows_segments(m, n) * kSearchMaxK entries each.
+long topk_rows_segments(long m, long n);
+hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out_d,
+                            int64_t *out_i, float *seg_d, uint32_t *seg_i, hipStream_t st);
+
+}  // namespace tq

@@ -1,0 +1,570 @@
+// tq_search.hip -- top-k search: the MFMA m x n scan with a fused top-k, run
+// as one persistent launch over a queue of (job, query tile, code range)
+// items. Jobs may come from different collections, dims and bit widths.
+//
+// Per item a workgroup walks its code tiles like k_mfma (tq_mfma.hip): 256
+// codes per tile, 32-coordinate chunks staged in LDS, fp16 hi+lo MFMA passes
+// and the reference's fp32 epilogue (space_turbo_quant.h:682-685), over a
+// query tile of 32 (QR = 1, two workgroups per CU) or 64 queries (QR = 2, one
+// workgroup per CU, room for 2 x 2 accumulator blocks). Instead of writing the
+// distances, each wavefront keeps the running best K (distance, code id)
+// entries of the codes it has seen in per-lane lists in registers: with
+// QR = 2 lane q holds query q's list; with QR = 1 lanes q and q + 32 hold
+// query q's lists for the two halves of the wavefront's 64 codes. A tile's
+// distances reach the lists through an LDS transpose; a wavefront-wide test
+// against the lists' K-th entries skips candidates no list takes. Every
+// (query, code) distance is the same arithmetic whatever else is in the
+// launch, and ties break by id, so a query's result does not depend on the
+// batch it was scored in.
+//
+// k_topk_merge then takes, per query, the best k of the lists of the items
+// that covered it. The reference has no top-k (its users run numpy
+// argpartition on the full distance matrix, python/benchmarks/
+// run_benchmark.py:251-255); ties there are unordered, here they are ordered
+// by id.
+#include "tq_mfma.h"
+#include "tq_mfma_dev.h"
+#include "tq_search.h"
+
+namespace tq {
+namespace {
+
+using namespace mfma;
+
+constexpr int kSN = kSearchNTile;  // codes per tile (64 per wavefront)
+constexpr int kSKC = 32;           // coordinates per chunk
+constexpr int kSARow = kSKC + 8;   // padded query-plane row, in halves
+constexpr int kSPlanes = 3;        // q_hi, q_lo, s_hi
+constexpr int kSCRow = 36;         // padded code row in bytes (8-bit chunks need 32)
+constexpr int kTRow = 65;          // padded transpose row, in floats
+
+__device__ __forceinline__ bool lex_less(float a, uint32_t ia, float b, uint32_t ib) {
+  return a < b || (a == b && ia < ib);
+}
+
+// Sorted insertion into a lane's ascending list (no-op when the entry is not
+// below the last one). Branch-free, so the wavefront runs it in lockstep.
+template <int K>
+__device__ __forceinline__ void topk_insert(float (&td)[K], uint32_t (&ti)[K], float d, uint32_t id) {
+  float cd = d;
+  uint32_t ci = id;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const bool less = lex_less(cd, ci, td[k], ti[k]);
+    const float nd = less ? td[k] : cd;
+    const uint32_t ni = less ? ti[k] : ci;
+    td[k] = less ? cd : td[k];
+    ti[k] = less ? ci : ti[k];
+    cd = nd;
+    ci = ni;
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void topk_clear(float (&td)[K], uint32_t (&ti)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    td[k] = __builtin_inff();
+    ti[k] = 0xFFFFFFFFu;
+  }
+}
+
+__global__ void k_reset_counter(int *counter) { *counter = 0; }
+
+template <int QR>
+struct SearchShape {
+  static constexpr int QT = 32 * QR;  // queries per item
+  static constexpr int WPE = 3 - QR;  // waves per SIMD the registers allow
+  static constexpr int STAGE = 2 * kSPlanes * QT * kSARow * 2 + 2 * kSN * kSCRow;
+  static constexpr int TRANS = 4 * 32 * kTRow * 4;
+  static constexpr int LDS = STAGE > TRANS ? STAGE : TRANS;
+  static constexpr int AUNITS = kSPlanes * QT * (kSKC / 8);  // 16-byte query-plane loads per chunk
+  static constexpr int ALOADS = (AUNITS + 255) / 256;        // per thread
+};
+
+template <int QR, int K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SearchShape<QR>::WPE,
+                                                                     SearchShape<QR>::WPE))) void
+k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ items,
+         const int *__restrict__ order, int nitems, int *__restrict__ counter,
+         float *__restrict__ cand_d, uint32_t *__restrict__ cand_i) {
+  using Sh = SearchShape<QR>;
+  constexpr int QT = Sh::QT;
+  constexpr int ALOADS = Sh::ALOADS;
+  // The chunk staging buffers and the per-tile transpose share one LDS area.
+  __shared__ __attribute__((aligned(16))) uint8_t lds[Sh::LDS];
+  __shared__ uint32_t l_tab[256];
+  __shared__ float l_qn[QT][2];
+  __shared__ int s_item;
+  auto a_lds = reinterpret_cast<_Float16(*)[kSPlanes][QT][kSARow]>(lds);
+  auto c_lds = reinterpret_cast<uint32_t(*)[kSN * kSCRow / 4]>(lds + 2 * kSPlanes * QT * kSARow * 2);
+  auto t_lds = reinterpret_cast<float(*)[32][kTRow]>(lds);
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int lr = lane & 31, kg = lane >> 5;
+
+  for (;;) {
+    __syncthreads();  // the previous item is done with every shared buffer
+    if (tid == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int pos = s_item;
+    if (pos >= nitems) break;  // every wavefront reads the same value and leaves together
+    const int it = order[pos];  // queue position -> item
+    const SearchItem item = items[it];
+    const SearchJobDev &job = jobs[item.job];
+    const SpaceDev s = job.s;
+    const bool nib = s.nib != 0;
+    const long m = job.m, n = job.n;
+    const long q0 = (long)item.qtile * QT;
+    const long mD = m * (long)s.D;
+    const int nchunks = s.D / kSKC;
+    const int cparts = nib ? 1 : 2;  // 16-byte loads per code and chunk
+
+    for (int i = tid; i < 256; i += blockDim.x) {
+      const float c = i < s.L ? s.cent[i] * kCScale : 0.0f;
+      const _Float16 ch = (_Float16)c;
+      const _Float16 cl = (_Float16)(c - (float)ch);
+      l_tab[i] = h_bits(ch) | (h_bits(cl) << 16);
+    }
+    if (tid < QT) {
+      const long q = q0 + tid;
+      const float *pq = job.prep + (q < m ? q : 0L) * prep_stride(s.D);
+      l_qn[tid][0] = pq[2 * s.D];
+      l_qn[tid][1] = pq[2 * s.D + 1];
+    }
+    PermTab ptab{};
+    if (nib) {
+      uint32_t bytes[4][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}};
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const float c = k < s.L ? s.cent[k] * kCScale : 0.0f;
+        const _Float16 chh = (_Float16)c;
+        const _Float16 cll = (_Float16)(c - (float)chh);
+        const uint32_t hb = h_bits(chh), lb = h_bits(cll);
+        const int word = k >> 2, sh = 8 * (k & 3);
+        bytes[0][word] |= (hb & 0xFFu) << sh;
+        bytes[1][word] |= (hb >> 8) << sh;
+        bytes[2][word] |= (lb & 0xFFu) << sh;
+        bytes[3][word] |= (lb >> 8) << sh;
+      }
+      ptab = PermTab{bytes[0][0], bytes[0][1], bytes[1][0], bytes[1][1],
+                     bytes[2][0], bytes[2][1], bytes[3][0], bytes[3][1]};
+    }
+
+    // Query-plane loads: 3 planes x QT rows x 32 fp16, 16 B each.
+    const _Float16 *asrc[ALOADS];
+    bool aok[ALOADS], ain[ALOADS];
+    int adst[ALOADS];
+#pragma unroll
+    for (int i = 0; i < ALOADS; ++i) {
+      const int idx0 = i * 256 + tid;
+      ain[i] = idx0 < Sh::AUNITS;  // with 32-query tiles the last round is half used
+      const int idx = ain[i] ? idx0 : 0;
+      const int p = idx / (QT * kSKC / 8);
+      const int row = (idx / (kSKC / 8)) % QT;
+      const int part = idx % (kSKC / 8);
+      const long q = q0 + row;
+      aok[i] = ain[i] && q < m;
+      asrc[i] = job.planes + p * mD + (aok[i] ? q : 0L) * s.D + 8 * part;
+      adst[i] = (p * QT + row) * kSARow + 8 * part;
+    }
+
+    float td[K];
+    uint32_t ti[K];
+    topk_clear<K>(td, ti);
+
+    for (int tile = item.t0; tile < item.t1; ++tile) {
+      const long j0 = (long)tile * kSN;
+      // Code loads: 256 codes x cparts x 16 B (1 or 2 per thread).
+      const uint8_t *csrc[2];
+      bool cok[2];
+      int cdst[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int idx = i * 256 + tid;
+        const int c = nib ? idx : idx >> 1;
+        const int part = nib ? 0 : idx & 1;
+        const long j = j0 + c;
+        cok[i] = (i < cparts) && j < n;
+        csrc[i] = job.codes + (cok[i] ? j : 0L) * (long)s.cs + 16 * part;
+        cdst[i] = (c * kSCRow + 16 * part) / 4;
+      }
+      uint4 ra[ALOADS], rc[2];
+      auto load_chunk = [&](int ch) {
+        const int k0 = ch * kSKC;
+#pragma unroll
+        for (int i = 0; i < ALOADS; ++i)
+          ra[i] = aok[i] ? *reinterpret_cast<const uint4 *>(asrc[i] + k0) : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          if (cok[i]) {
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(csrc[i] + (nib ? k0 / 2 : k0));
+            rc[i] = make_uint4(src[0], src[1], src[2], src[3]);
+          } else {
+            rc[i] = make_uint4(0u, 0u, 0u, 0u);
+          }
+        }
+      };
+      auto store_chunk = [&](int b) {
+        _Float16 *abuf = &a_lds[b][0][0][0];
+#pragma unroll
+        for (int i = 0; i < ALOADS; ++i)
+          if (ain[i]) *reinterpret_cast<uint4 *>(abuf + adst[i]) = ra[i];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          if (i < cparts) {
+            uint32_t *dst = &c_lds[b][cdst[i]];
+            dst[0] = rc[i].x;
+            dst[1] = rc[i].y;
+            dst[2] = rc[i].z;
+            dst[3] = rc[i].w;
+          }
+        }
+      };
+
+      f16v acc_m[QR][2], acc_q[QR][2];
+#pragma unroll
+      for (int r = 0; r < QR; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            acc_m[r][c][v] = 0.0f;
+            acc_q[r][c][v] = 0.0f;
+          }
+
+      __syncthreads();  // the previous tile's transpose reads are done
+      load_chunk(0);
+      store_chunk(0);
+      __syncthreads();
+      for (int ch = 0; ch < nchunks; ++ch) {
+        const int b = ch & 1;
+        if (ch + 1 < nchunks) load_chunk(ch + 1);
+#pragma unroll
+        for (int kb = 0; kb < kSKC / 16; ++kb) {
+          h8 a[QR][kSPlanes];
+#pragma unroll
+          for (int r = 0; r < QR; ++r)
+#pragma unroll
+            for (int p = 0; p < kSPlanes; ++p)
+              a[r][p] = *reinterpret_cast<const h8 *>(&a_lds[b][p][32 * r + lr][kb * 16 + 8 * kg]);
+          h8 bh[2], bl[2], bt[2];
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            const int code = wave * 64 + 32 * c + lr;
+            if (nib) {
+              decode_nib_word(c_lds[b][(code * kSCRow + kb * 8 + 4 * kg) / 4], ptab, bh[c], bl[c],
+                              bt[c]);
+            } else {
+              const uint32_t w0 = c_lds[b][(code * kSCRow + kb * 16 + 8 * kg) / 4];
+              const uint32_t w1 = c_lds[b][(code * kSCRow + kb * 16 + 8 * kg) / 4 + 1];
+              uint32_t u[8];
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                u[v] = (w0 >> (8 * v)) & 0xFFu;
+                u[4 + v] = (w1 >> (8 * v)) & 0xFFu;
+              }
+              decode8(u, l_tab, bh[c], bl[c], bt[c]);
+            }
+          }
+#define TQ_PASS(ACC, A, B)                                                               \
+  _Pragma("unroll") for (int r = 0; r < QR; ++r)                                         \
+      _Pragma("unroll") for (int c = 0; c < 2; ++c) ACC[r][c] =                          \
+          __builtin_amdgcn_mfma_f32_32x32x16_f16(a[r][A], B[c], ACC[r][c], 0, 0, 0);
+          TQ_PASS(acc_m, 0, bh)
+          TQ_PASS(acc_q, 2, bt)
+          TQ_PASS(acc_m, 1, bh)
+          TQ_PASS(acc_m, 0, bl)
+#undef TQ_PASS
+        }
+        if (ch + 1 < nchunks) store_chunk(b ^ 1);
+        __syncthreads();
+      }
+
+      // Distances (the reference epilogue, as in k_mfma) into the lists, one
+      // 32-query block at a time through the transpose area.
+      const long jbase = j0 + wave * 64;
+      float fm[2], fq[2], xnv[2], xns[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const long j = jbase + 32 * c + lr;
+        const float *meta =
+            reinterpret_cast<const float *>(job.codes + (j < n ? j : 0L) * (long)s.cs + s.packed);
+        const float xn = meta[0], gamma = meta[1], sigma = meta[2];
+        fm[c] = sigma * kMseUnscale;
+        fq[c] = s.scale * gamma * kQjlUnscale;
+        xnv[c] = xn;
+        xns[c] = xn * xn;
+      }
+#pragma unroll
+      for (int r = 0; r < QR; ++r) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int row = 8 * (v >> 2) + 4 * kg + (v & 3);
+            const int qrow = 32 * r + row;
+            const float ip = (acc_m[r][c][v] * fm[c] + acc_q[r][c][v] * fq[c]) * xnv[c] * l_qn[qrow][1];
+            const float d = l_qn[qrow][0] + xns[c] - 2.0f * ip;
+            t_lds[wave][row][32 * c + lr] = d > 0.0f ? d : 0.0f;
+          }
+        __syncthreads();
+        // QR = 2: lanes 32r .. 32r+31 own this block's queries and read all
+        // 64 codes. QR = 1: lane q and q + 32 read codes 0-31 and 32-63 of
+        // query q. Codes past n and lanes that own nothing here offer the
+        // empty entry (inf, ~0), which no list takes.
+        const int h = QR == 1 ? (lane >> 5) : 0;
+        const bool mine = QR == 1 || (lane >> 5) == r;
+        const float *rowp = &t_lds[wave][lane & 31][32 * h];
+        const long cbase = jbase + 32 * h;
+        if (jbase < n) {
+          for (int k = 0; k < 64 / (3 - QR); ++k) {
+            const bool ok = mine && cbase + k < n;
+            const float d = ok ? rowp[k] : __builtin_inff();
+            const uint32_t id = ok ? (uint32_t)(cbase + k) : 0xFFFFFFFFu;
+            if (__any(lex_less(d, id, td[K - 1], ti[K - 1]))) topk_insert<K>(td, ti, d, id);
+          }
+        }
+        __syncthreads();
+      }
+    }
+
+    // This wavefront's lists for the item.
+    const long base = (((long)it * 4 + wave) * 64 + lane) * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      cand_d[base + k] = td[k];
+      cand_i[base + k] = ti[k];
+    }
+  }
+}
+
+// Lane-held sorted lists -> the wavefront's overall best k_out, ascending:
+// k_out rounds of a wavefront-wide (distance, id) minimum; the lane holding
+// the minimum pops its head. Lane 0 writes the entries (id ~0 -> -1).
+template <int K>
+__device__ void wave_pop_topk(float (&td)[K], uint32_t (&ti)[K], int k_out, int lane,
+                              float *out_d, int64_t *out_i) {
+  for (int k = 0; k < k_out; ++k) {
+    float bd = td[0];
+    uint32_t bi = ti[0];
+    for (int o = 32; o > 0; o >>= 1) {
+      const float od = __shfl_xor(bd, o);
+      const uint32_t oi = __shfl_xor(bi, o);
+      if (lex_less(od, oi, bd, bi)) {
+        bd = od;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      out_d[k] = bd;
+      out_i[k] = bi == 0xFFFFFFFFu ? -1 : (int64_t)bi;
+    }
+    // Ids are unique within a query's lists, so at most one lane holds a
+    // real minimum; empty entries popping together is harmless.
+    if (td[0] == bd && ti[0] == bi) {
+#pragma unroll
+      for (int t = 0; t + 1 < K; ++t) {
+        td[t] = td[t + 1];
+        ti[t] = ti[t + 1];
+      }
+      td[K - 1] = __builtin_inff();
+      ti[K - 1] = 0xFFFFFFFFu;
+    }
+  }
+}
+
+// Sorted list of K entries at (d, i) into the lane's list; stops at the first
+// entry the list does not take.
+template <int K>
+__device__ __forceinline__ void take_sorted(float (&td)[K], uint32_t (&ti)[K], const float *d,
+                                            const uint32_t *i) {
+  for (int k = 0; k < K; ++k) {
+    const float dk = d[k];
+    const uint32_t ik = i[k];
+    if (!lex_less(dk, ik, td[K - 1], ti[K - 1])) break;
+    topk_insert<K>(td, ti, dk, ik);
+  }
+}
+
+// One wavefront per query: lanes take the query's candidate lists (4 or 8
+// per item that covered it), then pop the best k_out. Waves are laid out as
+// 64 per group; a group with 32-query tiles leaves half of them idle.
+template <int K>
+__global__ __launch_bounds__(256) void k_topk_merge(const SearchJobDev *__restrict__ jobs,
+                                                    const SearchGroup *__restrict__ groups,
+                                                    int ngroups, int k_out,
+                                                    const float *__restrict__ cand_d,
+                                                    const uint32_t *__restrict__ cand_i) {
+  const int lane = threadIdx.x & 63;
+  const long wq = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long g = wq >> 6;
+  if (g >= ngroups) return;
+  const int qi = (int)(wq & 63);
+  const SearchGroup grp = groups[g];
+  const int qt = 32 * grp.qr;
+  if (qi >= qt) return;
+  const SearchJobDev &job = jobs[grp.job];
+  const long q = (long)grp.qtile * qt + qi;
+  if (q >= job.m) return;
+  float td[K];
+  uint32_t ti[K];
+  topk_clear<K>(td, ti);
+  const int per_item = grp.qr == 1 ? 8 : 4;  // lists per item for this query
+  const int nlists = (grp.iend - grp.ibeg) * per_item;
+  for (int l = lane; l < nlists; l += 64) {
+    const long it = grp.ibeg + l / per_item;
+    const int r = l % per_item;
+    const int w = grp.qr == 1 ? r >> 1 : r;
+    const int src_lane = grp.qr == 1 ? qi + 32 * (r & 1) : qi;
+    const long base = ((it * 4 + w) * 64 + src_lane) * K;
+    take_sorted<K>(td, ti, cand_d + base, cand_i + base);
+  }
+  wave_pop_topk<K>(td, ti, k_out, lane, job.out_d + q * k_out, job.out_i + q * k_out);
+}
+
+// Row-wise top-k of a dense distance matrix, stage 1: workgroup (seg, row)
+// scans columns [seg*span, (seg+1)*span) of the row, each lane keeping the
+// best K of a strided subset; the 4 wavefronts' 256 lists are merged through
+// LDS by wavefront 0 into one sorted list per (row, seg).
+template <int K>
+__global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dist, long n,
+                                                   long span, float *__restrict__ seg_d,
+                                                   uint32_t *__restrict__ seg_i) {
+  __shared__ float s_d[256 * K];
+  __shared__ uint32_t s_i[256 * K];
+  const int tid = threadIdx.x;
+  const long row = blockIdx.y;
+  const long c0 = (long)blockIdx.x * span;
+  const long c1 = c0 + span < n ? c0 + span : n;
+  const float *p = dist + row * n;
+  float td[K];
+  uint32_t ti[K];
+  topk_clear<K>(td, ti);
+  for (long j = c0 + tid; j < c1; j += 256) {
+    const float d = p[j];
+    if (lex_less(d, (uint32_t)j, td[K - 1], ti[K - 1])) topk_insert<K>(td, ti, d, (uint32_t)j);
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    s_d[tid * K + k] = td[k];
+    s_i[tid * K + k] = ti[k];
+  }
+  __syncthreads();
+  if (tid >= 64) return;
+  topk_clear<K>(td, ti);
+  for (int l = tid; l < 256; l += 64) take_sorted<K>(td, ti, s_d + l * K, s_i + l * K);
+  const long base = (row * gridDim.x + blockIdx.x) * K;
+  // K pops write the segment's sorted list (ids as u32, ~0 = empty).
+  for (int k = 0; k < K; ++k) {
+    float bd = td[0];
+    uint32_t bi = ti[0];
+    for (int o = 32; o > 0; o >>= 1) {
+      const float od = __shfl_xor(bd, o);
+      const uint32_t oi = __shfl_xor(bi, o);
+      if (lex_less(od, oi, bd, bi)) {
+        bd = od;
+        bi = oi;
+      }
+    }
+    if (tid == 0) {
+      seg_d[base + k] = bd;
+      seg_i[base + k] = bi;
+    }
+    if (td[0] == bd && ti[0] == bi) {
+#pragma unroll
+      for (int t = 0; t + 1 < K; ++t) {
+        td[t] = td[t + 1];
+        ti[t] = ti[t + 1];
+      }
+      td[K - 1] = __builtin_inff();
+      ti[K - 1] = 0xFFFFFFFFu;
+    }
+  }
+}
+
+// Stage 2: one wavefront per row merges the row's segment lists.
+template <int K>
+__global__ __launch_bounds__(256) void k_topk_rows_merge(long m, int nseg, int k_out,
+                                                         const float *__restrict__ seg_d,
+                                                         const uint32_t *__restrict__ seg_i,
+                                                         float *__restrict__ out_d,
+                                                         int64_t *__restrict__ out_i) {
+  const int lane = threadIdx.x & 63;
+  const long row = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= m) return;
+  float td[K];
+  uint32_t ti[K];
+  topk_clear<K>(td, ti);
+  for (int l = lane; l < nseg; l += 64) {
+    const long base = (row * nseg + l) * K;
+    take_sorted<K>(td, ti, seg_d + base, seg_i + base);
+  }
+  wave_pop_topk<K>(td, ti, k_out, lane, out_d + row * k_out, out_i + row * k_out);
+}
+
+}  // namespace
+
+int search_max_k() { return kSearchMaxK; }
+
+hipError_t launch_search(const SearchJobDev *d_jobs, const SearchItem *d_items,
+                         const int *d_order, const int *nitems_qr, const SearchGroup *d_groups,
+                         int ngroups, int k, int *d_counter, float *d_cand_d,
+                         uint32_t *d_cand_i, const int *grid_qr, hipStream_t st) {
+  if (k < 1 || k > kSearchMaxK) return hipErrorInvalidValue;
+  // Items with 32-query tiles come first in `order`, then the 64-query ones;
+  // each kind drains in its own persistent launch.
+  int off = 0;
+  for (int qr = 1; qr <= 2; ++qr) {
+    const int cnt = nitems_qr[qr - 1];
+    if (cnt <= 0) continue;
+    if (grid_qr[qr - 1] < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, d_counter + (qr - 1));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (qr == 1)
+      hipLaunchKernelGGL((k_search<1, kSearchMaxK>), dim3((unsigned)grid_qr[0]), dim3(256), 0, st,
+                         d_jobs, d_items, d_order + off, cnt, d_counter, d_cand_d, d_cand_i);
+    else
+      hipLaunchKernelGGL((k_search<2, kSearchMaxK>), dim3((unsigned)grid_qr[1]), dim3(256), 0, st,
+                         d_jobs, d_items, d_order + off, cnt, d_counter + 1, d_cand_d, d_cand_i);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    off += cnt;
+  }
+  if (ngroups <= 0) return hipSuccess;
+  const long waves = (long)ngroups * 64;
+  hipLaunchKernelGGL(k_topk_merge<kSearchMaxK>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0,
+                     st, d_jobs, d_groups, ngroups, k, d_cand_d, d_cand_i);
+  return hipGetLastError();
+}
+
+long topk_rows_segments(long m, long n) {
+  // About 2048 workgroups over the matrix, each segment at least 4096 columns
+  // (so m * segments < 2048 + m).
+  long seg = (2048 + m - 1) / m;
+  const long max_seg = (n + 4095) / 4096;
+  if (seg > max_seg) seg = max_seg;
+  if (seg < 1) seg = 1;
+  if (seg > 65535) seg = 65535;
+  return seg;
+}
+
+hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out_d,
+                            int64_t *out_i, float *seg_d, uint32_t *seg_i, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  if (k < 1 || k > kSearchMaxK || m > 65535 || n >= 0xFFFFFFFFL) return hipErrorInvalidValue;
+  const long nseg = topk_rows_segments(m, n);
+  const long span = (n + nseg - 1) / nseg;
+  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3((unsigned)nseg, (unsigned)m), dim3(256), 0, st,
+                     dist, n, span, seg_d, seg_i);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_topk_rows_merge<kSearchMaxK>, dim3((unsigned)((m + 3) / 4)), dim3(256), 0,
+                     st, m, (int)nseg, k, seg_d, seg_i, out_d, out_i);
+  return hipGetLastError();
+}
+
+}  // namespace tq

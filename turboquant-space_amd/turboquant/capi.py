@@ -27,8 +27,11 @@ EXPORTED = [
     "tq_last_error", "tq_device_count", "tq_space_create", "tq_space_destroy",
     "tq_space_info_get", "tq_space_tables", "tq_encode_batch", "tq_prepare_queries",
     "tq_distance_1_to_n", "tq_distance_m_to_n", "tq_distance_prepared",
-    "tq_distance_m_to_n_symmetric",
+    "tq_distance_m_to_n_symmetric", "tq_search_plan_create", "tq_search_plan_run",
+    "tq_search_plan_run_graph", "tq_search_plan_tune", "tq_search_plan_stats",
+    "tq_search_plan_destroy", "tq_search", "tq_topk_rows",
 ]
+TQ_SEARCH_MAX_K = 16
 
 
 class SpaceInfo(ctypes.Structure):
@@ -43,6 +46,22 @@ class SpaceInfo(ctypes.Structure):
         ("padded", ctypes.c_int32),
         ("prepared_stride", ctypes.c_uint64),
     ]
+
+
+class SearchJob(ctypes.Structure):
+    _fields_ = [
+        ("space", ctypes.c_void_p),
+        ("queries", ctypes.c_void_p),
+        ("m", ctypes.c_uint64),
+        ("codes", ctypes.c_void_p),
+        ("n", ctypes.c_uint64),
+        ("out_dist", ctypes.c_void_p),
+        ("out_ids", ctypes.c_void_p),
+    ]
+
+
+class SearchTuning(ctypes.Structure):
+    _fields_ = [("query_tile", ctypes.c_int32), ("items_per_wg", ctypes.c_int32)]
 
 
 class TQError(RuntimeError):
@@ -85,6 +104,24 @@ def lib():
         L.tq_distance_prepared.argtypes = [P, P, U64, P, U64, P, I, P]
         L.tq_distance_m_to_n_symmetric.restype = I
         L.tq_distance_m_to_n_symmetric.argtypes = [P, P, U64, P, U64, P, I, P]
+        JP = ctypes.POINTER(SearchJob)
+        TP = ctypes.POINTER(SearchTuning)
+        L.tq_search_plan_create.restype = I
+        L.tq_search_plan_create.argtypes = [JP, U64, I, I, TP, ctypes.POINTER(P)]
+        L.tq_search_plan_run.restype = I
+        L.tq_search_plan_run.argtypes = [P, P]
+        L.tq_search_plan_run_graph.restype = I
+        L.tq_search_plan_run_graph.argtypes = [P, P]
+        L.tq_search_plan_tune.restype = I
+        L.tq_search_plan_tune.argtypes = [P, P, TP]
+        L.tq_search_plan_stats.restype = I
+        L.tq_search_plan_stats.argtypes = [P, ctypes.POINTER(U64), ctypes.POINTER(U64)]
+        L.tq_search_plan_destroy.restype = I
+        L.tq_search_plan_destroy.argtypes = [P]
+        L.tq_search.restype = I
+        L.tq_search.argtypes = [JP, U64, I, I, P]
+        L.tq_topk_rows.restype = I
+        L.tq_topk_rows.argtypes = [P, U64, U64, I, P, P, P]
         _lib = L
     return _lib
 
@@ -152,3 +189,58 @@ class Space:
     def symmetric(self, A, m, B, n, out, mem=TQ_MEM_DEVICE, stream=None):
         check(lib().tq_distance_m_to_n_symmetric(self.h, ptr(A), m, ptr(B), n, ptr(out), mem,
                                                  stream))
+
+
+def make_jobs(jobs):
+    """ctypes array of SearchJob from (space, queries, m, codes, n, out_dist,
+    out_ids) tuples; buffers are device tensors or raw addresses."""
+    arr = (SearchJob * len(jobs))()
+    for i, (sp, q, m, c, n, od, oi) in enumerate(jobs):
+        arr[i] = SearchJob(sp.h.value if isinstance(sp, Space) else sp, ptr(q), m, ptr(c), n,
+                           ptr(od), ptr(oi))
+    return arr
+
+
+class SearchPlan:
+    """Owns a tq_search_plan over device-resident job buffers."""
+
+    def __init__(self, jobs, k, mode=TQ_MODE_MFMA, query_tile=0, items_per_wg=0):
+        self._jobs = make_jobs(jobs)
+        self._keep = jobs  # the spaces must outlive the plan
+        tun = SearchTuning(query_tile, items_per_wg)
+        h = ctypes.c_void_p()
+        check(lib().tq_search_plan_create(self._jobs, len(jobs), k, mode, ctypes.byref(tun),
+                                          ctypes.byref(h)))
+        self.h = h
+        self.k = k
+
+    def run(self, stream=None):
+        check(lib().tq_search_plan_run(self.h, stream))
+
+    def run_graph(self, stream=None):
+        check(lib().tq_search_plan_run_graph(self.h, stream))
+
+    def tune(self, stream=None):
+        t = SearchTuning()
+        check(lib().tq_search_plan_tune(self.h, stream, ctypes.byref(t)))
+        return int(t.query_tile), int(t.items_per_wg)
+
+    def stats(self):
+        items, wgs = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().tq_search_plan_stats(self.h, ctypes.byref(items), ctypes.byref(wgs)))
+        return int(items.value), int(wgs.value)
+
+    def close(self):
+        if self.h:
+            lib().tq_search_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def topk_rows(dist, m, n, k, out_dist, out_ids, stream=None):
+    check(lib().tq_topk_rows(ptr(dist), m, n, k, ptr(out_dist), ptr(out_ids), stream))
