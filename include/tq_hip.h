@@ -181,6 +181,68 @@ int tq_search(const tq_search_job *jobs, uint64_t njobs, int k, int mode, void *
 int tq_topk_rows(const float *dist, uint64_t m, uint64_t n, int k, float *out_dist,
                  int64_t *out_ids, void *stream);
 
+/* ---- serving ----------------------------------------------------------------
+ * No reference counterpart (the reference is a library of synchronous calls).
+ * A server keeps code collections resident in device memory and runs search
+ * and append requests submitted from any thread. Its scheduler batches
+ * continuously: requests that arrive while earlier batches are on the device
+ * are admitted as soon as a batch slot and its workspace allow, and all
+ * searches of a batch run as one search plan (one persistent launch for
+ * TQ_MODE_MFMA). Requests take effect in submission order: a search sees the
+ * codes of exactly the appends submitted before it. */
+typedef struct tq_server_s *tq_server;
+
+typedef struct tq_server_config {
+  int32_t mode;               /* TQ_MODE_MFMA or TQ_MODE_FP32 */
+  int32_t k;                  /* top-k of every search, 1..TQ_SEARCH_MAX_K */
+  uint64_t hbm_budget;        /* device bytes: collections + batch slots */
+  uint64_t slot_bytes;        /* device workspace per batch slot (0: 1 GiB) */
+  uint32_t max_inflight;      /* batch slots, i.e. batches on the device at once (0: 2) */
+  uint32_t max_batch_queries; /* queries per batch (0: 4096) */
+} tq_server_config;
+
+typedef struct tq_server_stats {
+  uint64_t batches;
+  uint64_t searches_done;
+  uint64_t appends_done;
+  uint64_t queries_done;
+  uint64_t failed;
+  uint64_t resident_bytes;  /* collection capacity held in device memory */
+  uint64_t reserved_bytes;  /* batch slot workspaces */
+  uint64_t max_batch_requests;
+} tq_server_stats;
+
+/* Starts the scheduler; batch slots are charged to hbm_budget up front. */
+int tq_server_create(const tq_server_config *cfg, tq_server *out);
+/* Completes every submitted request, then stops. */
+int tq_server_destroy(tq_server s);
+/* An empty resident collection of up to `capacity` codes of `space`
+ * (capacity * code_size device bytes from the budget). */
+int tq_server_add_collection(tq_server s, tq_space space, uint64_t capacity,
+                             uint32_t *coll);
+/* Encodes n vectors (host float32 [n, dim]) and appends their codes. X must
+ * stay valid until the ticket completes. Fails at submission (TQ_ENOMEM)
+ * when the collection would exceed its capacity. */
+int tq_server_submit_append(tq_server s, uint32_t coll, const float *X, uint64_t n,
+                            uint64_t *ticket);
+/* Top-k search of m queries (host float32 [m, dim]); results are written to
+ * host out_dist / out_ids [m, k] before the ticket completes. Q must stay
+ * valid until then. */
+int tq_server_submit_search(tq_server s, uint32_t coll, const float *Q, uint64_t m,
+                            float *out_dist, int64_t *out_ids, uint64_t *ticket);
+/* Blocks until the ticket completes and releases it. Returns the request's
+ * status; latency_s (may be NULL) receives submission-to-completion time. */
+int tq_server_wait(tq_server s, uint64_t ticket, double *latency_s);
+/* Codes in the collection counting every submitted append, and the device
+ * address of its code rows. */
+int tq_server_collection(tq_server s, uint32_t coll, uint64_t *n, const uint8_t **codes);
+/* Copies code rows [start, start + count) to host memory (the reference's
+ * byte layout, e.g. to persist the collection). The rows must come from
+ * appends whose tickets have completed. */
+int tq_server_read_codes(tq_server s, uint32_t coll, uint64_t start, uint64_t count,
+                         uint8_t *host);
+int tq_server_stats_get(tq_server s, tq_server_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,4 +28,14 @@ int hip_fail(hipError_t e, const char *what);
 int cu_count(int device);
 const char *last_error();
 }  // namespace capi
+
+// Search plans without their own memory (tq_plan.hip): plan_new checks the
+// jobs and lays out the workspace on the host; plan_bind places it in caller
+// memory of at least plan_ws_bytes and enqueues the table uploads on `st`;
+// plan_enqueue is tq_search_plan_run. Destroy with tq_search_plan_destroy.
+int plan_new(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
+             const tq_search_tuning *tuning, tq_search_plan *out);
+size_t plan_ws_bytes(tq_search_plan p);
+int plan_bind(tq_search_plan p, void *ws, size_t bytes, hipStream_t st);
+int plan_enqueue(tq_search_plan p, hipStream_t st);
 }  // namespace tq

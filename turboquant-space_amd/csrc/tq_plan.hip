@@ -128,9 +128,16 @@ struct tq_search_plan_s {
   int grid_qr[2] = {0, 0};
   long gather_max = 0, scatter_max = 0;
 
-  // device workspace (one allocation)
+  // workspace layout (byte offsets) of the current build
+  size_t o_jobs = 0, o_items = 0, o_order = 0, o_groups = 0, o_gather = 0, o_scatter = 0,
+         o_counter = 0, o_cand_d = 0, o_cand_i = 0, o_all_d = 0, o_all_i = 0, o_dist = 0,
+         o_seg_d = 0, o_seg_i = 0, o_scan_ws = 0, scan_ws_bytes = 0, need = 0;
+  std::vector<size_t> o_q, o_prep, o_planes;
+
+  // device workspace: owned (tq_search_plan_create) or lent by the caller
   void *ws = nullptr;
   size_t ws_bytes = 0;
+  bool owned = false;
   tq::SearchJobDev *d_jobs = nullptr;
   tq::SearchItem *d_items = nullptr;
   int *d_order = nullptr;
@@ -160,9 +167,10 @@ struct tq_search_plan_s {
     graph = nullptr;
   }
   void drop_ws() {
-    if (ws) (void)hipFree(ws);
+    if (ws && owned) (void)hipFree(ws);
     ws = nullptr;
     ws_bytes = 0;
+    owned = false;
   }
   ~tq_search_plan_s() {
     drop_graph();
@@ -197,9 +205,9 @@ tq_search_tuning knobs_for(const tq_search_plan_s *p, const PlanBatch &b) {
   return t;
 }
 
-// Work queue, groups and copy tables for the current knobs; then the
-// workspace (grown if needed) and the upload of the tables.
-int build(tq_search_plan_s *p) {
+// Work queue, groups and the workspace layout for the current knobs (host
+// only; bind() places it in device memory).
+int layout(tq_search_plan_s *p) {
   const char *what = "tq_search_plan";
   p->drop_graph();
   p->h_jobs.clear();
@@ -294,27 +302,29 @@ int build(tq_search_plan_s *p) {
     return at;
   };
   const size_t nb = p->batches.size();
-  const size_t o_jobs = take(nb * sizeof(tq::SearchJobDev));
-  const size_t o_items = take(p->h_items.size() * sizeof(tq::SearchItem));
-  const size_t o_order = take(p->h_order.size() * sizeof(int));
-  const size_t o_groups = take(p->h_groups.size() * sizeof(tq::SearchGroup));
-  const size_t o_gather = take(p->jobs.size() * sizeof(tq::CopySeg));
-  const size_t o_scatter = take(p->jobs.size() * sizeof(tq::OutSeg));
-  const size_t o_counter = take(2 * sizeof(int));
+  p->o_jobs = take(nb * sizeof(tq::SearchJobDev));
+  p->o_items = take(p->h_items.size() * sizeof(tq::SearchItem));
+  p->o_order = take(p->h_order.size() * sizeof(int));
+  p->o_groups = take(p->h_groups.size() * sizeof(tq::SearchGroup));
+  p->o_gather = take(p->jobs.size() * sizeof(tq::CopySeg));
+  p->o_scatter = take(p->jobs.size() * sizeof(tq::OutSeg));
+  p->o_counter = take(2 * sizeof(int));
   const size_t cand = p->h_items.size() * 4 * 64 * (size_t)tq::kSearchMaxK;
-  const size_t o_cand_d = take(cand * sizeof(float));
-  const size_t o_cand_i = take(cand * sizeof(uint32_t));
-  const size_t o_all_d = take((size_t)p->rows * p->k * sizeof(float));
-  const size_t o_all_i = take((size_t)p->rows * p->k * sizeof(int64_t));
-  std::vector<size_t> o_q(nb), o_prep(nb), o_planes(nb);
+  p->o_cand_d = take(cand * sizeof(float));
+  p->o_cand_i = take(cand * sizeof(uint32_t));
+  p->o_all_d = take((size_t)p->rows * p->k * sizeof(float));
+  p->o_all_i = take((size_t)p->rows * p->k * sizeof(int64_t));
+  p->o_q.assign(nb, 0);
+  p->o_prep.assign(nb, 0);
+  p->o_planes.assign(nb, 0);
   size_t dist_elems = 0, seg_elems = 0, scan_ws = 0;
   for (size_t bi = 0; bi < nb; ++bi) {
     const PlanBatch &b = p->batches[bi];
     const tq_space_info &info = b.space->info;
-    o_q[bi] = take((size_t)b.m * info.dim * sizeof(float));
-    o_prep[bi] = take((size_t)b.m * info.prepared_stride * sizeof(float));
+    p->o_q[bi] = take((size_t)b.m * info.dim * sizeof(float));
+    p->o_prep[bi] = take((size_t)b.m * info.prepared_stride * sizeof(float));
     if (p->mode == TQ_MODE_MFMA) {
-      o_planes[bi] = take(3 * (size_t)b.m * info.padded_dim * sizeof(_Float16));
+      p->o_planes[bi] = take(3 * (size_t)b.m * info.padded_dim * sizeof(_Float16));
     } else if (b.m > 0) {
       const long rows = fp32_block_rows(b);
       dist_elems = std::max(dist_elems, (size_t)rows * b.n);
@@ -323,41 +333,49 @@ int build(tq_search_plan_s *p) {
       scan_ws = std::max(scan_ws, tq::scan_fp32_ws_bytes(b.space->dev, rows));
     }
   }
-  const size_t o_dist = take(dist_elems * sizeof(float));
-  const size_t o_seg_d = take(seg_elems * sizeof(float));
-  const size_t o_seg_i = take(seg_elems * sizeof(uint32_t));
-  const size_t o_scan_ws = take(scan_ws);
+  p->o_dist = take(dist_elems * sizeof(float));
+  p->o_seg_d = take(seg_elems * sizeof(float));
+  p->o_seg_i = take(seg_elems * sizeof(uint32_t));
+  p->o_scan_ws = take(scan_ws);
+  p->scan_ws_bytes = scan_ws;
+  p->need = off;
+  return TQ_OK;
+}
 
-  hipError_t e = hipSetDevice(p->device);
-  if (e != hipSuccess) return hip_fail(e, what);
-  if (off > p->ws_bytes) {
-    p->drop_ws();
-    e = hipMalloc(&p->ws, off);
-    if (e != hipSuccess) return hip_fail(e, what);
-    p->ws_bytes = off;
-  }
-  char *w = static_cast<char *>(p->ws);
-  p->d_jobs = reinterpret_cast<tq::SearchJobDev *>(w + o_jobs);
-  p->d_items = reinterpret_cast<tq::SearchItem *>(w + o_items);
-  p->d_order = reinterpret_cast<int *>(w + o_order);
-  p->d_groups = reinterpret_cast<tq::SearchGroup *>(w + o_groups);
-  p->d_gather = reinterpret_cast<tq::CopySeg *>(w + o_gather);
-  p->d_scatter = reinterpret_cast<tq::OutSeg *>(w + o_scatter);
-  p->d_counter = reinterpret_cast<int *>(w + o_counter);
-  p->cand_d = reinterpret_cast<float *>(w + o_cand_d);
-  p->cand_i = reinterpret_cast<uint32_t *>(w + o_cand_i);
-  p->all_d = reinterpret_cast<float *>(w + o_all_d);
-  p->all_i = reinterpret_cast<int64_t *>(w + o_all_i);
-  p->dist = reinterpret_cast<float *>(w + o_dist);
-  p->seg_d = reinterpret_cast<float *>(w + o_seg_d);
-  p->seg_i = reinterpret_cast<uint32_t *>(w + o_seg_i);
-  p->scan_ws = scan_ws ? static_cast<void *>(w + o_scan_ws) : nullptr;
+// Places the layout at `ws` (at least p->need bytes), fills the device-side
+// tables and uploads them: with hipMemcpy, or enqueued on `st` when `async`
+// (the plan's host tables must then live until the copies have run).
+int bind(tq_search_plan_s *p, void *ws, hipStream_t st, bool async) {
+  const char *what = "tq_search_plan";
+  p->drop_graph();
+  char *w = static_cast<char *>(ws);
+  p->d_jobs = reinterpret_cast<tq::SearchJobDev *>(w + p->o_jobs);
+  p->d_items = reinterpret_cast<tq::SearchItem *>(w + p->o_items);
+  p->d_order = reinterpret_cast<int *>(w + p->o_order);
+  p->d_groups = reinterpret_cast<tq::SearchGroup *>(w + p->o_groups);
+  p->d_gather = reinterpret_cast<tq::CopySeg *>(w + p->o_gather);
+  p->d_scatter = reinterpret_cast<tq::OutSeg *>(w + p->o_scatter);
+  p->d_counter = reinterpret_cast<int *>(w + p->o_counter);
+  p->cand_d = reinterpret_cast<float *>(w + p->o_cand_d);
+  p->cand_i = reinterpret_cast<uint32_t *>(w + p->o_cand_i);
+  p->all_d = reinterpret_cast<float *>(w + p->o_all_d);
+  p->all_i = reinterpret_cast<int64_t *>(w + p->o_all_i);
+  p->dist = reinterpret_cast<float *>(w + p->o_dist);
+  p->seg_d = reinterpret_cast<float *>(w + p->o_seg_d);
+  p->seg_i = reinterpret_cast<uint32_t *>(w + p->o_seg_i);
+  p->scan_ws = p->scan_ws_bytes ? static_cast<void *>(w + p->o_scan_ws) : nullptr;
 
+  p->h_jobs.clear();
+  p->h_gather.clear();
+  p->h_scatter.clear();
+  p->gather_max = p->scatter_max = 0;
+  const size_t nb = p->batches.size();
   for (size_t bi = 0; bi < nb; ++bi) {
     PlanBatch &b = p->batches[bi];
-    b.queries = reinterpret_cast<float *>(w + o_q[bi]);
-    b.prep = reinterpret_cast<float *>(w + o_prep[bi]);
-    b.planes = p->mode == TQ_MODE_MFMA ? reinterpret_cast<_Float16 *>(w + o_planes[bi]) : nullptr;
+    b.queries = reinterpret_cast<float *>(w + p->o_q[bi]);
+    b.prep = reinterpret_cast<float *>(w + p->o_prep[bi]);
+    b.planes =
+        p->mode == TQ_MODE_MFMA ? reinterpret_cast<_Float16 *>(w + p->o_planes[bi]) : nullptr;
     tq::SearchJobDev jd{};
     jd.s = b.space->dev;
     jd.planes = b.planes;
@@ -380,8 +398,11 @@ int build(tq_search_plan_s *p) {
       p->scatter_max = std::max(p->scatter_max, (long)j.m * p->k);
     }
   }
+  hipError_t e = hipSuccess;
   auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
-    return bytes ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    if (!bytes) return hipSuccess;
+    return async ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st)
+                 : hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
   };
   if ((e = up(p->d_jobs, p->h_jobs.data(), nb * sizeof(tq::SearchJobDev))) != hipSuccess ||
       (e = up(p->d_items, p->h_items.data(), p->h_items.size() * sizeof(tq::SearchItem))) !=
@@ -395,6 +416,27 @@ int build(tq_search_plan_s *p) {
           hipSuccess)
     return hip_fail(e, what);
   return TQ_OK;
+}
+
+// layout() + bind() into memory the plan owns (grown when the layout needs
+// more than it has).
+int build(tq_search_plan_s *p) {
+  const char *what = "tq_search_plan";
+  if (int r = layout(p)) return r;
+  hipError_t e = hipSetDevice(p->device);
+  if (e != hipSuccess) return hip_fail(e, what);
+  if (!p->owned || p->need > p->ws_bytes) {
+    p->drop_graph();
+    p->drop_ws();
+    e = hipMalloc(&p->ws, p->need);
+    if (e != hipSuccess) {
+      p->ws = nullptr;
+      return hip_fail(e, what);
+    }
+    p->ws_bytes = p->need;
+    p->owned = true;
+  }
+  return bind(p, p->ws, nullptr, false);
 }
 
 unsigned copy_blocks(long max_count) {
@@ -454,10 +496,12 @@ int enqueue(tq_search_plan_s *p, hipStream_t st) {
 
 }  // namespace
 
-extern "C" {
+namespace tq {
 
-int tq_search_plan_create(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
-                          const tq_search_tuning *tuning, tq_search_plan *out) {
+// Checks the jobs and builds the plan's host side and layout; no device
+// memory is touched. Shared by tq_search_plan_create and the server.
+int plan_new(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
+             const tq_search_tuning *tuning, tq_search_plan *out) {
   const char *what = "tq_search_plan_create";
   if (!out) return fail(TQ_EINVAL, "%s: out is NULL", what);
   *out = nullptr;
@@ -522,7 +566,39 @@ int tq_search_plan_create(const tq_search_job *jobs, uint64_t njobs, int k, int 
     if (mode == TQ_MODE_MFMA && b.m > (1L << 31) - 64)
       return delete p, fail(TQ_EINVAL, "%s: too many queries for one collection", what);
   }
-  hipError_t e = hipSetDevice(device);
+  if (int r = layout(p)) return delete p, r;
+  *out = p;
+  return TQ_OK;
+}
+
+size_t plan_ws_bytes(tq_search_plan p) { return p->need; }
+
+// Binds the plan to caller memory (>= plan_ws_bytes) and enqueues the table
+// uploads on `st`; the plan must outlive that work.
+int plan_bind(tq_search_plan p, void *ws, size_t bytes, hipStream_t st) {
+  if (bytes < p->need)
+    return fail(TQ_EINVAL, "tq_search_plan: workspace of %zu bytes, %zu needed", bytes, p->need);
+  p->drop_ws();
+  p->ws = ws;
+  p->ws_bytes = bytes;
+  p->owned = false;
+  return bind(p, ws, st, true);
+}
+
+int plan_enqueue(tq_search_plan p, hipStream_t st) { return enqueue(p, st); }
+
+}  // namespace tq
+
+extern "C" {
+
+int tq_search_plan_create(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
+                          const tq_search_tuning *tuning, tq_search_plan *out) {
+  const char *what = "tq_search_plan_create";
+  if (!out) return fail(TQ_EINVAL, "%s: out is NULL", what);
+  *out = nullptr;
+  tq_search_plan p = nullptr;
+  if (int r = tq::plan_new(jobs, njobs, k, mode, tuning, &p)) return r;
+  hipError_t e = hipSetDevice(p->device);
   if (e != hipSuccess) return delete p, hip_fail(e, what);
   if (int r = build(p)) return delete p, r;
   *out = p;

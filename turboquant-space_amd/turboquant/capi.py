@@ -29,7 +29,10 @@ EXPORTED = [
     "tq_distance_1_to_n", "tq_distance_m_to_n", "tq_distance_prepared",
     "tq_distance_m_to_n_symmetric", "tq_search_plan_create", "tq_search_plan_run",
     "tq_search_plan_run_graph", "tq_search_plan_tune", "tq_search_plan_stats",
-    "tq_search_plan_destroy", "tq_search", "tq_topk_rows",
+    "tq_search_plan_destroy", "tq_search", "tq_topk_rows", "tq_server_create",
+    "tq_server_destroy", "tq_server_add_collection", "tq_server_submit_append",
+    "tq_server_submit_search", "tq_server_wait", "tq_server_collection", "tq_server_read_codes",
+    "tq_server_stats_get",
 ]
 TQ_SEARCH_MAX_K = 16
 
@@ -62,6 +65,23 @@ class SearchJob(ctypes.Structure):
 
 class SearchTuning(ctypes.Structure):
     _fields_ = [("query_tile", ctypes.c_int32), ("items_per_wg", ctypes.c_int32)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int32),
+        ("k", ctypes.c_int32),
+        ("hbm_budget", ctypes.c_uint64),
+        ("slot_bytes", ctypes.c_uint64),
+        ("max_inflight", ctypes.c_uint32),
+        ("max_batch_queries", ctypes.c_uint32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "batches", "searches_done", "appends_done", "queries_done", "failed", "resident_bytes",
+        "reserved_bytes", "max_batch_requests")]
 
 
 class TQError(RuntimeError):
@@ -122,6 +142,25 @@ def lib():
         L.tq_search.argtypes = [JP, U64, I, I, P]
         L.tq_topk_rows.restype = I
         L.tq_topk_rows.argtypes = [P, U64, U64, I, P, P, P]
+        U32 = ctypes.c_uint32
+        L.tq_server_create.restype = I
+        L.tq_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]
+        L.tq_server_destroy.restype = I
+        L.tq_server_destroy.argtypes = [P]
+        L.tq_server_add_collection.restype = I
+        L.tq_server_add_collection.argtypes = [P, P, U64, ctypes.POINTER(U32)]
+        L.tq_server_submit_append.restype = I
+        L.tq_server_submit_append.argtypes = [P, U32, P, U64, ctypes.POINTER(U64)]
+        L.tq_server_submit_search.restype = I
+        L.tq_server_submit_search.argtypes = [P, U32, P, U64, P, P, ctypes.POINTER(U64)]
+        L.tq_server_wait.restype = I
+        L.tq_server_wait.argtypes = [P, U64, ctypes.POINTER(ctypes.c_double)]
+        L.tq_server_collection.restype = I
+        L.tq_server_collection.argtypes = [P, U32, ctypes.POINTER(U64), ctypes.POINTER(P)]
+        L.tq_server_read_codes.restype = I
+        L.tq_server_read_codes.argtypes = [P, U32, U64, U64, P]
+        L.tq_server_stats_get.restype = I
+        L.tq_server_stats_get.argtypes = [P, ctypes.POINTER(ServerStats)]
         _lib = L
     return _lib
 
@@ -244,3 +283,66 @@ class SearchPlan:
 
 def topk_rows(dist, m, n, k, out_dist, out_ids, stream=None):
     check(lib().tq_topk_rows(ptr(dist), m, n, k, ptr(out_dist), ptr(out_ids), stream))
+
+
+class Server:
+    """Owns a tq_server. Inputs and outputs are host numpy arrays, which the
+    caller keeps alive until the request's ticket is waited for."""
+
+    def __init__(self, mode=TQ_MODE_MFMA, k=10, hbm_budget=8 << 30, slot_bytes=0,
+                 max_inflight=0, max_batch_queries=0):
+        cfg = ServerConfig(mode, k, hbm_budget, slot_bytes, max_inflight, max_batch_queries)
+        h = ctypes.c_void_p()
+        check(lib().tq_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.k = k
+        self._spaces = []
+
+    def add_collection(self, space, capacity):
+        c = ctypes.c_uint32()
+        check(lib().tq_server_add_collection(self.h, space.h, capacity, ctypes.byref(c)))
+        self._spaces.append(space)
+        return int(c.value)
+
+    def append(self, coll, X):
+        t = ctypes.c_uint64()
+        check(lib().tq_server_submit_append(self.h, coll, ptr(X), len(X), ctypes.byref(t)))
+        return int(t.value)
+
+    def search(self, coll, Q, out_d, out_i):
+        t = ctypes.c_uint64()
+        check(lib().tq_server_submit_search(self.h, coll, ptr(Q), len(Q), ptr(out_d), ptr(out_i),
+                                            ctypes.byref(t)))
+        return int(t.value)
+
+    def wait(self, ticket):
+        lat = ctypes.c_double()
+        check(lib().tq_server_wait(self.h, ticket, ctypes.byref(lat)))
+        return float(lat.value)
+
+    def collection(self, coll):
+        n, codes = ctypes.c_uint64(), ctypes.c_void_p()
+        check(lib().tq_server_collection(self.h, coll, ctypes.byref(n), ctypes.byref(codes)))
+        return int(n.value), codes.value
+
+    def read_codes(self, coll, start, count, code_size):
+        import numpy as np
+        out = np.empty((count, code_size), np.uint8)
+        check(lib().tq_server_read_codes(self.h, coll, start, count, ptr(out)))
+        return out
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().tq_server_stats_get(self.h, ctypes.byref(st)))
+        return {name: int(getattr(st, name)) for name, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().tq_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
