@@ -72,6 +72,209 @@ def measured_traffic(kernel):
         return None
 
 
+# Batched uneven launch: 5 resident collections, (dim, bits) and code sizes
+# 96/4 -> 76 B, 128/8 -> 140 B, 384/4 -> 268 B, 768/8 -> 1036 B, 1536/4 -> 1036 B.
+BATCH_COLLS = [(96, 4), (128, 8), (384, 4), (768, 8), (1536, 4)]
+BATCH_N = 500_000
+BATCH_COUNT = 256
+
+
+def uneven_batches(seed=15, count=BATCH_COUNT, mmax=512):
+    """m ~ floor(exp(U(0, ln 513))) capped at 512 and a uniform target
+    collection per batch, both from one numpy generator."""
+    rng = np.random.default_rng(seed)
+    ms = np.minimum(np.floor(np.exp(rng.uniform(0.0, np.log(mmax + 1), count))), mmax)
+    targets = rng.integers(0, len(BATCH_COLLS), count)
+    return [(int(t), int(m)) for t, m in zip(targets, ms)]
+
+
+def bench_batched(torch, capi, stream, sh, reps=5):
+    """256 query batches of uneven m against 5 collections of 500,000 codes:
+    one persistent search plan (MFMA, fused top-10, tuned, HIP-graph replay)
+    against the same batches as per-batch plain launches (prepare, fp32
+    distance_m_to_n, row top-10) and per-batch MFMA launches."""
+    colls = []
+    for i, (dim, bits) in enumerate(BATCH_COLLS):
+        sp = capi.Space(dim, bits)
+        codes = torch.empty((BATCH_N, sp.cs), dtype=torch.uint8, device="cuda")
+        centres = None
+        for start, x, centres in gaussian_mixture(torch, BATCH_N, dim, 512, 0.3, 10 + i,
+                                                  chunk=250_000):
+            sp.encode(x, x.shape[0], codes[start:start + x.shape[0]], stream=sh)
+        colls.append((sp, codes, centres))
+    batches = uneven_batches()
+    jobs, queries = [], []
+    for b, (t, m) in enumerate(batches):
+        sp, codes, centres = colls[t]
+        _, q, _ = next(gaussian_mixture(torch, m, sp.dim, 512, 0.3, 1000 + b, centres=centres))
+        od = torch.empty((m, 10), dtype=torch.float32, device="cuda")
+        oi = torch.empty((m, 10), dtype=torch.int64, device="cuda")
+        queries.append(q)
+        jobs.append((sp, q, m, codes, BATCH_N, od, oi))
+    torch.cuda.synchronize()
+    total_q = sum(m for _, m in batches)
+    dists = total_q * BATCH_N
+    flops = sum(4.0 * colls[t][0].D * m * BATCH_N for t, m in batches)
+
+    out = {"batches": len(batches), "queries": total_q, "codes_per_collection": BATCH_N,
+           "collections": [{"dim": d, "bits": b} for d, b in BATCH_COLLS], "k": 10}
+    plan = capi.SearchPlan(jobs, 10, capi.TQ_MODE_MFMA)
+    t0 = time.perf_counter()
+    qt, ipw = plan.tune(stream=sh)
+    out["tune_s"] = time.perf_counter() - t0
+    items, wgs = plan.stats()
+    plan.run_graph(stream=sh)
+    ms = event_time(torch, stream, lambda: plan.run_graph(stream=sh), reps)
+    out["persistent_mfma"] = {"ms": ms, "dist_per_s": dists / (ms / 1e3),
+                              "queries_per_s": total_q / (ms / 1e3),
+                              "tflops": flops / (ms / 1e3) / 1e12, "work_items": items,
+                              "workgroups": wgs, "tuned": {"query_tile": qt, "items_per_wg": ipw}}
+    ref_ids = [j[6].clone() for j in jobs]
+    plan.close()
+    log(f"batched: persistent MFMA graph {ms:.2f} ms ({dists / (ms / 1e3) / 1e9:.1f} G dist/s)")
+
+    plan32 = capi.SearchPlan(jobs, 10, capi.TQ_MODE_FP32)
+    plan32.run_graph(stream=sh)
+    ms32 = event_time(torch, stream, lambda: plan32.run_graph(stream=sh), max(1, reps // 2))
+    out["persistent_fp32"] = {"ms": ms32, "dist_per_s": dists / (ms32 / 1e3)}
+    plan32.close()
+    log(f"batched: merged fp32 plan {ms32:.2f} ms")
+
+    # Per-batch launches through the plain entry points.
+    mmax = max(m for _, m in batches)
+    dist = torch.empty((mmax, BATCH_N), dtype=torch.float32, device="cuda")
+    prep = torch.empty((mmax, max(c[0].prep_stride for c in colls)), dtype=torch.float32,
+                       device="cuda")
+
+    def per_batch(mode):
+        for (t, m), q, j in zip(batches, queries, jobs):
+            sp, codes, _ = colls[t]
+            sp.prepare(q, m, prep, stream=sh)
+            sp.scan(prep, m, codes, BATCH_N, dist, mode=mode, stream=sh)
+            capi.topk_rows(dist, m, BATCH_N, 10, j[5], j[6], stream=sh)
+
+    for name, mode in (("per_batch_fp32", capi.TQ_MODE_FP32), ("per_batch_mfma", capi.TQ_MODE_MFMA)):
+        per_batch(mode)
+        msb = event_time(torch, stream, lambda: per_batch(mode), 1)
+        out[name] = {"ms": msb, "dist_per_s": dists / (msb / 1e3)}
+        log(f"batched: {name} {msb:.2f} ms")
+    torch.cuda.synchronize()
+    # The MFMA full matrices give the same top-10 ids as the fused search.
+    out["per_batch_mfma_ids_equal_fused"] = all(torch.equal(a, j[6]) for a, j in zip(ref_ids, jobs))
+    out["speedup_vs_per_batch_fp32"] = out["per_batch_fp32"]["ms"] / ms
+    del dist, prep, colls, jobs, queries
+    torch.cuda.empty_cache()
+    return out
+
+
+# Serving: 4 resident collections of 10,000,000 codes at dim 768, 4-bit.
+SERVE_COLLS = 4
+SERVE_N = 10_000_000
+
+
+def bench_serving(torch, capi, sh, qps_by_mode, seconds, n_per_coll=SERVE_N, seed=21):
+    """Poisson arrivals at a fixed rate; each request a top-10 search of
+    m ~ floor(exp(U(0, ln 257))) queries against a uniformly chosen
+    collection, or with probability 0.05 an append of 10,000 vectors. Latency
+    is submission to results-in-host-memory, measured by the server."""
+    import threading
+    results = {}
+    sp = capi.Space(DIM, BITS)
+    # Collections are generated once on the device and loaded into each server.
+    staged = []
+    for c in range(SERVE_COLLS):
+        codes = torch.empty((n_per_coll, sp.cs), dtype=torch.uint8, device="cuda")
+        centres = None
+        for start, x, centres in gaussian_mixture(torch, n_per_coll, DIM, 8192, 0.3, 20 + 100 * c):
+            sp.encode(x, x.shape[0], codes[start:start + x.shape[0]], stream=sh)
+        staged.append((codes, centres))
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(seed)
+    horizon = max(seconds * max(qps_by_mode.values()) * 1.5, 50)
+    # Request pool, drawn once: queries and append payloads on the host.
+    reqs = []
+    for i in range(int(horizon)):
+        c = int(rng.integers(0, SERVE_COLLS))
+        if rng.random() < 0.05:
+            reqs.append(("append", c, 10_000))
+        else:
+            m = int(min(np.floor(np.exp(rng.uniform(0.0, np.log(257)))), 256))
+            reqs.append(("search", c, m))
+    payload = {}
+    for i, (kind, c, rows) in enumerate(reqs):
+        _, x, _ = next(gaussian_mixture(torch, rows, DIM, 8192, 0.3, 5000 + i,
+                                        centres=staged[c][1]))
+        payload[i] = np.ascontiguousarray(x.cpu().numpy())
+    for mode_name, qps in qps_by_mode.items():
+        mode = capi.TQ_MODE_MFMA if mode_name == "mfma" else capi.TQ_MODE_FP32
+        cap = n_per_coll + 20 * 10_000 * 2
+        srv = capi.Server(mode=mode, k=10, hbm_budget=96 << 30, slot_bytes=2 << 30,
+                          max_inflight=2, max_batch_queries=4096)
+        ids = []
+        for c in range(SERVE_COLLS):
+            ids.append(srv.add_collection(sp, cap))
+            srv.load_codes(ids[-1], staged[c][0], n_per_coll)
+        arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / qps, len(reqs)))
+        count = int(np.searchsorted(arr, seconds))
+        tickets = [None] * count
+        outs = {}
+        done = []
+        lock = threading.Condition()
+
+        def waiter():
+            for i in range(count):
+                with lock:
+                    while tickets[i] is None:
+                        lock.wait()
+                lat = srv.wait(tickets[i])
+                done.append((i, lat, time.perf_counter()))
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t_start = time.perf_counter()
+        for i in range(count):
+            delay = t_start + arr[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            kind, c, rows = reqs[i]
+            if kind == "append":
+                t = srv.append(ids[c], payload[i])
+            else:
+                od = np.empty((rows, 10), np.float32)
+                oi = np.empty((rows, 10), np.int64)
+                outs[i] = (od, oi)
+                t = srv.search(ids[c], payload[i], od, oi)
+            with lock:
+                tickets[i] = t
+                lock.notify()
+        th.join()
+        t_end = max(d[2] for d in done) if done else time.perf_counter()
+        st = srv.stats()
+        srv.close()
+        lat_s = np.array([d[1] for d in done if reqs[d[0]][0] == "search"])
+        lat_a = np.array([d[1] for d in done if reqs[d[0]][0] == "append"])
+        nq = sum(reqs[i][2] for i in range(count) if reqs[i][0] == "search")
+        wall = t_end - t_start
+        results[mode_name] = {
+            "qps_offered": qps, "requests": count, "searches": int(len(lat_s)),
+            "appends": int(len(lat_a)), "seconds": wall,
+            "qps_achieved": count / wall, "queries_per_s": nq / wall,
+            "dist_per_s": nq * n_per_coll / wall,
+            "p50_ms": float(np.percentile(lat_s, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat_s, 99) * 1e3),
+            "mean_ms": float(lat_s.mean() * 1e3),
+            "append_p50_ms": float(np.percentile(lat_a, 50) * 1e3) if len(lat_a) else None,
+            "batches": st["batches"], "max_batch_requests": st["max_batch_requests"],
+            "failed": st["failed"]}
+        log(f"serving {mode_name} @ {qps} QPS: {count} requests, p50 "
+            f"{results[mode_name]['p50_ms']:.1f} ms, p99 {results[mode_name]['p99_ms']:.1f} ms, "
+            f"{st['batches']} batches")
+    del staged
+    torch.cuda.empty_cache()
+    return {"collections": SERVE_COLLS, "codes_per_collection": n_per_coll, "dim": DIM,
+            "bits": BITS, "k": 10, "modes": results}
+
+
 def cpu_baseline(codes_host, Q_host, budget_s=8.0):
     """The reference CPU build (oracle/_ref, compiled in place from the
     reference sources with its Release flags) on this host's cores:
@@ -119,6 +322,14 @@ def main():
     ap.add_argument("--codes", type=int, default=N_CODES)
     ap.add_argument("--queries", type=int, default=M_QUERIES)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched uneven-launch measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-seconds", type=float, default=12.0)
+    ap.add_argument("--qps-mfma", type=float, default=40.0)
+    ap.add_argument("--qps-fp32", type=float, default=4.0)
+    ap.add_argument("--serve-codes", type=int, default=SERVE_N,
+                    help="codes per resident collection (4 collections)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one 1 GiB device-to-device copy first (a kernel of known "
                          "traffic, used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -223,6 +434,16 @@ def main():
         if cpu:
             log(f"cpu baseline: {cpu['value'] / 1e6:.2f} M dist/s on {cpu['cores']} threads")
 
+    # The collection and output of the headline measurement are released
+    # before the batched and serving runs build their own.
+    del codes, out, one
+    torch.cuda.empty_cache()
+    batched = None if args.no_batched else bench_batched(torch, capi, stream, sh)
+    serving = None
+    if not args.no_serving:
+        serving = bench_serving(torch, capi, sh, {"mfma": args.qps_mfma, "fp32": args.qps_fp32},
+                                args.serve_seconds, n_per_coll=args.serve_codes)
+
     line = {
         "metric": "distances/s at dim768 4-bit, MFMA vs fp32 GPU path; p99 latency @ fixed QPS",
         "value": r["dist_per_s"], "unit": "distances/s", "n_gpus": 1, "steps": args.steps,
@@ -238,6 +459,8 @@ def main():
         "mfma_path_dist_per_s": results.get("mfma", {}).get("dist_per_s"),
         "distance_1_to_n": one_to_n,
         "encode_vec_per_s": enc_vps,
+        "batched_uneven": batched,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     return 0

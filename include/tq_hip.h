@@ -236,6 +236,12 @@ int tq_server_wait(tq_server s, uint64_t ticket, double *latency_s);
 /* Codes in the collection counting every submitted append, and the device
  * address of its code rows. */
 int tq_server_collection(tq_server s, uint32_t coll, uint64_t *n, const uint8_t **codes);
+/* Appends n existing codes (the reference's byte layout; host or device
+ * memory per `mem`) to the collection, e.g. to load a persisted collection.
+ * Synchronous: searches submitted after it returns see the rows; none may be
+ * submitted on this collection while it runs. */
+int tq_server_load_codes(tq_server s, uint32_t coll, const uint8_t *codes, uint64_t n,
+                         int mem);
 /* Copies code rows [start, start + count) to host memory (the reference's
  * byte layout, e.g. to persist the collection). The rows must come from
  * appends whose tickets have completed. */

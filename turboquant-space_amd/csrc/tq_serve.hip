@@ -546,6 +546,35 @@ int tq_server_collection(tq_server s, uint32_t coll, uint64_t *n, const uint8_t 
   return TQ_OK;
 }
 
+int tq_server_load_codes(tq_server s, uint32_t coll, const uint8_t *codes, uint64_t n,
+                         int mem) {
+  const char *what = "tq_server_load_codes";
+  if (!s || (n && !codes)) return fail(TQ_EINVAL, "%s: NULL argument", what);
+  if (mem != TQ_MEM_HOST && mem != TQ_MEM_DEVICE)
+    return fail(TQ_EINVAL, "%s: mem must be TQ_MEM_HOST or TQ_MEM_DEVICE", what);
+  uint8_t *dst = nullptr;
+  size_t cs = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (coll >= s->colls.size()) return fail(TQ_EINVAL, "%s: no collection %u", what, coll);
+    Collection &c = s->colls[coll];
+    if (c.n + n > c.capacity)
+      return fail(TQ_ENOMEM, "%s: collection %u holds %llu of %llu codes; %llu more do not fit",
+                  what, coll, (unsigned long long)c.n, (unsigned long long)c.capacity,
+                  (unsigned long long)n);
+    cs = c.space->info.code_size;
+    dst = c.codes + c.n * cs;
+    c.n += n;
+  }
+  if (!n) return TQ_OK;
+  hipError_t e = hipSetDevice(s->device);
+  if (e == hipSuccess)
+    e = hipMemcpy(dst, codes, n * cs,
+                  mem == TQ_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, what);
+  return TQ_OK;
+}
+
 int tq_server_read_codes(tq_server s, uint32_t coll, uint64_t start, uint64_t count,
                          uint8_t *host) {
   const char *what = "tq_server_read_codes";

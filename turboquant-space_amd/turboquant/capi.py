@@ -31,7 +31,7 @@ EXPORTED = [
     "tq_search_plan_run_graph", "tq_search_plan_tune", "tq_search_plan_stats",
     "tq_search_plan_destroy", "tq_search", "tq_topk_rows", "tq_server_create",
     "tq_server_destroy", "tq_server_add_collection", "tq_server_submit_append",
-    "tq_server_submit_search", "tq_server_wait", "tq_server_collection", "tq_server_read_codes",
+    "tq_server_submit_search", "tq_server_wait", "tq_server_collection", "tq_server_load_codes", "tq_server_read_codes",
     "tq_server_stats_get",
 ]
 TQ_SEARCH_MAX_K = 16
@@ -157,6 +157,8 @@ def lib():
         L.tq_server_wait.argtypes = [P, U64, ctypes.POINTER(ctypes.c_double)]
         L.tq_server_collection.restype = I
         L.tq_server_collection.argtypes = [P, U32, ctypes.POINTER(U64), ctypes.POINTER(P)]
+        L.tq_server_load_codes.restype = I
+        L.tq_server_load_codes.argtypes = [P, U32, P, U64, I]
         L.tq_server_read_codes.restype = I
         L.tq_server_read_codes.argtypes = [P, U32, U64, U64, P]
         L.tq_server_stats_get.restype = I
@@ -324,6 +326,9 @@ class Server:
         n, codes = ctypes.c_uint64(), ctypes.c_void_p()
         check(lib().tq_server_collection(self.h, coll, ctypes.byref(n), ctypes.byref(codes)))
         return int(n.value), codes.value
+
+    def load_codes(self, coll, codes, n, mem=TQ_MEM_DEVICE):
+        check(lib().tq_server_load_codes(self.h, coll, ptr(codes), n, mem))
 
     def read_codes(self, coll, start, count, code_size):
         import numpy as np
