@@ -59,6 +59,8 @@ typedef struct tq_space_info {
   float scale;           /* sqrt(pi/2)/sqrt(padded_dim)                  :325 */
   int32_t padded;        /* dim != padded_dim                            :318 */
   uint64_t prepared_stride; /* floats per prepared query (tq_prepare_queries) */
+  uint64_t sym_prepared_stride; /* floats per prepared symmetric code
+                                   (tq_prepare_symmetric) */
 } tq_space_info;
 
 /* Thread-local text of the last error ("" if none). */
@@ -112,11 +114,46 @@ int tq_distance_prepared(tq_space s, const float *prepared, uint64_t m,
 
 /* distanceBatchMToNSymmetric -- space_turbo_quant.h:602-615;
  * bindings.cpp:197-218 (and distance_symmetric :145-153 with m = n = 1).
- * MSE-only estimator (the QJL bits and gamma are not used). */
+ * MSE-only estimator (the QJL bits and gamma are not used); fp32. */
 int tq_distance_m_to_n_symmetric(tq_space s, const uint8_t *codes_a,
                                  uint64_t m, const uint8_t *codes_b,
                                  uint64_t n, float *out, int mem,
                                  void *stream);
+
+/* Symmetric estimator variants. */
+#define TQ_SYM_MSE 0   /* distBuildScalar(B4)           space_turbo_quant.h:688-714, :985-1019 */
+#define TQ_SYM_FULL 1  /* distBuildFullScalar           :731-811 */
+#define TQ_SYM_LIGHT 2 /* distBuildLightScalar          :893-945 */
+
+/* distanceBatchMToNSymmetric / ...SymmetricFull / ...SymmetricLight
+ * (:602-647; bindings.cpp:197-266). mode TQ_MODE_MFMA is available for
+ * TQ_SYM_MSE (the decoded a codes as fp16 hi+lo queries, error <= 1e-4 of
+ * the fp64 estimator); the QJL-corrected variants run in fp32. */
+int tq_distance_m_to_n_symmetric_ex(tq_space s, const uint8_t *codes_a,
+                                    uint64_t m, const uint8_t *codes_b,
+                                    uint64_t n, float *out, int variant,
+                                    int mode, int mem, void *stream);
+
+/* prepareSymmetricQuery -- space_turbo_quant.h:498-532 (bindings.cpp:276-282).
+ * m rows of sym_prepared_stride floats: the a side of the full estimator
+ * (reconstruction, transposed-Hadamard signs, Hadamard of the reconstruction
+ * with the sign term folded in, norm^2, norm). */
+int tq_prepare_symmetric(tq_space s, const uint8_t *codes, uint64_t m,
+                         float *prepared, int mem, void *stream);
+
+/* distanceSymmetricFullPrepared / distanceBatch1ToNSymmetricFull /
+ * distanceBatchMToNSymmetricFullPrepared -- :537-572 (bindings.cpp:284-334):
+ * full symmetric distances of m prepared codes against n codes, [m, n]. */
+int tq_distance_symmetric_prepared(tq_space s, const float *prepared,
+                                   uint64_t m, const uint8_t *codes,
+                                   uint64_t n, float *out, int mem,
+                                   void *stream);
+
+/* rotatedCoordsBatch -- space_turbo_quant.h:411-435 (bindings.cpp:119-132):
+ * [n, padded_dim] float32 sigma-normalised rotated coordinates, bit-identical
+ * to the reference. */
+int tq_rotated_coords_batch(tq_space s, const float *X, uint64_t n, float *out,
+                            int mem, void *stream);
 
 /* ---- top-k search ---------------------------------------------------------
  * No reference counterpart: the reference returns full distance matrices and

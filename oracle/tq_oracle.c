@@ -424,6 +424,51 @@ float tqo_distance_symmetric_full(const tqo_space *s, const uint8_t *a, const ui
   return d > 0.0f ? d : 0.0f;
 }
 
+/* prepareSymmetricQuery (space_turbo_quant.h:498-532) + the prepared full
+ * distance (distBuildFullPreparedScalar, :818-891): the query side is
+ * reconstructed and Hadamard-transformed once; all four dot products run in
+ * one loop over the post-Hadamard vectors. */
+float tqo_distance_symmetric_full_prepared(const tqo_space *s, const uint8_t *a,
+                                           const uint8_t *b) {
+  const size_t D = s->dim;
+  float ma[3], mb[3];
+  code_meta(s, a, ma);
+  code_meta(s, b, mb);
+  float *rq = (float *)malloc(D * sizeof(float)), *sq = (float *)malloc(D * sizeof(float));
+  float *rb = (float *)malloc(D * sizeof(float));
+  for (size_t i = 0; i < D; ++i) {
+    unsigned u = code_unit(s, a, i);
+    rq[i] = s->centroids[u >> 1] * ma[2];
+    sq[i] = (u & 1) ? 1.0f : -1.0f;
+    rb[i] = s->centroids[code_unit(s, b, i) >> 1] * mb[2];
+  }
+  tqo_rht(rq, s->qjl_signs, D);
+  tqo_rht(rb, s->qjl_signs, D);
+  const float scale_gamma_q = s->scale * ma[1];
+  const float pi_over_2d = (float)M_PI / (2.0f * (float)D);
+  const float pi_over_2d_gamma_q = pi_over_2d * ma[1];
+  float d_rr = 0.0f, d_rs = 0.0f, d_sr = 0.0f, d_ss = 0.0f;
+  for (size_t i = 0; i < D; ++i) {
+    float sb = (code_unit(s, b, i) & 1) ? 1.0f : -1.0f;
+    d_rr += rq[i] * rb[i];
+    d_rs += rq[i] * sb;
+    d_sr += sq[i] * rb[i];
+    d_ss += sq[i] * sb;
+  }
+  float t2 = s->scale * mb[1] * d_rs;
+  float t3 = scale_gamma_q * d_sr;
+  float t4 = pi_over_2d_gamma_q * mb[1] * d_ss;
+  float corr = t2 + t3 + t4;
+  float mc = fabsf(d_rr);
+  corr = corr < -mc ? -mc : (corr > mc ? mc : corr);
+  float ip = (d_rr + corr) * ma[0] * mb[0];
+  float d = ma[0] * ma[0] + mb[0] * mb[0] - 2.0f * ip;
+  free(rq);
+  free(sq);
+  free(rb);
+  return d > 0.0f ? d : 0.0f;
+}
+
 /* space_turbo_quant.h:876-924 */
 float tqo_distance_symmetric_light(const tqo_space *s, const uint8_t *a, const uint8_t *b) {
   const size_t D = s->dim;

@@ -72,6 +72,8 @@ void tqo_distance_m_to_n_symmetric(const tqo_space *s, const uint8_t *A, size_t 
 /* Symmetric with QJL correction (space_turbo_quant.h:731-800, :876-924). */
 float tqo_distance_symmetric_full(const tqo_space *s, const uint8_t *a, const uint8_t *b);
 float tqo_distance_symmetric_light(const tqo_space *s, const uint8_t *a, const uint8_t *b);
+float tqo_distance_symmetric_full_prepared(const tqo_space *s, const uint8_t *a,
+                                           const uint8_t *b);
 
 /* The same estimators evaluated in double precision from the same codes
  * (query preparation in double as well). */

@@ -55,6 +55,8 @@ def olib():
         L.tqo_distance_symmetric_full.argtypes = [P, P, P]
         L.tqo_distance_symmetric_light.restype = ctypes.c_float
         L.tqo_distance_symmetric_light.argtypes = [P, P, P]
+        L.tqo_distance_symmetric_full_prepared.restype = ctypes.c_float
+        L.tqo_distance_symmetric_full_prepared.argtypes = [P, P, P]
         L.tqo_distance_m_to_n_f64.argtypes = [P, P, SZ, P, SZ, P]
         L.tqo_distance_m_to_n_symmetric_f64.argtypes = [P, P, SZ, P, SZ, P]
         _o = L

@@ -55,6 +55,7 @@ def load_ref():
     L.tqref_distance_m_to_n_symmetric.argtypes = [P, P, SZ, P, SZ, P]
     L.tqref_distance_m_to_n_symmetric_full.argtypes = [P, P, SZ, P, SZ, P]
     L.tqref_distance_m_to_n_symmetric_light.argtypes = [P, P, SZ, P, SZ, P]
+    L.tqref_distance_m_to_n_symmetric_full_prepared.argtypes = [P, P, SZ, P, SZ, P]
     L.tqref_codebook.argtypes = [ctypes.c_int, P, P]
     L.tqref_rotated_coords_batch.argtypes = [P, P, SZ, P]
     L.tqref_signs.argtypes = [SZ, ctypes.c_uint64, P]
@@ -126,6 +127,9 @@ def main():
         syml = np.zeros((4, len(X)), np.float32)
         L.tqref_distance_m_to_n_symmetric_light(h, ptr(codes[-4:]), 4, ptr(codes), len(X),
                                                 ptr(syml))
+        symfp = np.zeros((4, len(X)), np.float32)
+        L.tqref_distance_m_to_n_symmetric_full_prepared(h, ptr(codes[-4:]), 4, ptr(codes),
+                                                        len(X), ptr(symfp))
         rot = np.zeros((len(X), D), np.float32)
         L.tqref_rotated_coords_batch(h, ptr(X), len(X), ptr(rot))
         tag = f"{dim}_{bits}"
@@ -139,6 +143,7 @@ def main():
         out[f"sym_{tag}"] = sym
         out[f"symfull_{tag}"] = symf
         out[f"symlight_{tag}"] = syml
+        out[f"symfullprep_{tag}"] = symfp
         shapes_meta.append({"dim": dim, "bits": bits, "padded_dim": int(D), "code_size": int(cs),
                             "rows": int(len(X)), "queries": m})
         L.tqref_destroy(h)

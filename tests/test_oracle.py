@@ -215,6 +215,11 @@ def test_symmetric_full_and_light_match_reference_fixture(dim, bits):
     xns = o.trailer(codes)[:, 0].astype(np.float64) ** 2
     assert scale_rel_err(full, GOLD[f"symfull_{tag}"], na, xns).max() <= 1e-5
     assert scale_rel_err(light, GOLD[f"symlight_{tag}"], na, xns).max() <= 1e-5
+    prep = np.array([[olib().tqo_distance_symmetric_full_prepared(o.h, ptr(a), ptr(b))
+                      for b in codes] for a in A], np.float32)
+    assert scale_rel_err(prep, GOLD[f"symfullprep_{tag}"], na, xns).max() <= 1e-5
+    # The prepared form differs from the unprepared one by rounding only.
+    assert scale_rel_err(GOLD[f"symfullprep_{tag}"], GOLD[f"symfull_{tag}"], na, xns).max() <= 1e-5
 
 
 @pytest.mark.parametrize("dim,bits", [(128, 4), (768, 4), (768, 8), (1000, 4)])

@@ -1,0 +1,174 @@
+"""GPU parity of the non-hot reference methods: symmetric distances with QJL
+correction (full, light, prepared full), the MFMA form of the symmetric
+scan, rotated coordinates, and the drop-in module's methods for them.
+
+Bars: rotated coordinates bit-identical to the reference build; fp32
+distances within 1e-5 of (||a||^2 + ||b||^2) of the reference build's
+fixtures and of the oracle; the symmetric MFMA form within 1e-4 of the same
+scale of the FP64 restatement.
+"""
+import json
+import sys
+
+import numpy as np
+import pytest
+
+from _oracle import MANIFEST, Oracle, load_golden, olib, ptr, scale_rel_err
+
+pytestmark = pytest.mark.gpu
+
+GOLD = load_golden()
+with open(MANIFEST) as f:
+    META = json.load(f)
+SHAPES = [(s["dim"], s["bits"]) for s in META["shapes"]]
+FP32_TOL = 1e-5
+MFMA_TOL = 1e-4
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def host_sym(sp, A, B, variant, mode=0):
+    out = np.zeros((len(A), len(B)), np.float32)
+    sp.symmetric(np.ascontiguousarray(A), len(A), np.ascontiguousarray(B), len(B), out,
+                 mem=0, variant=variant, mode=mode)
+    return out
+
+
+def oracle_pairs(fn, o, A, B):
+    return np.array([[fn(o.h, ptr(a), ptr(b)) for b in B] for a in A], np.float32)
+
+
+@pytest.mark.parametrize("dim,bits", SHAPES)
+def test_golden_full_light_prepared_rotated(gpu, dim, bits):
+    say(f"[gpu] sym qjl golden dim={dim} bits={bits}")
+    capi = gpu
+    sp = capi.Space(dim, bits)
+    o = Oracle(dim, bits)
+    tag = f"{dim}_{bits}"
+    codes = np.ascontiguousarray(GOLD[f"codes_{tag}"])
+    A = np.ascontiguousarray(codes[-4:])
+    na = o.trailer(A)[:, 0].astype(np.float64) ** 2
+    xns = o.trailer(codes)[:, 0].astype(np.float64) ** 2
+    full = host_sym(sp, A, codes, capi.TQ_SYM_FULL)
+    light = host_sym(sp, A, codes, capi.TQ_SYM_LIGHT)
+    assert scale_rel_err(full, GOLD[f"symfull_{tag}"], na, xns).max() <= FP32_TOL
+    assert scale_rel_err(light, GOLD[f"symlight_{tag}"], na, xns).max() <= FP32_TOL
+    prep = np.zeros((4, sp.sym_prep_stride), np.float32)
+    sp.prepare_symmetric(A, 4, prep, mem=capi.TQ_MEM_HOST)
+    fp = np.zeros((4, len(codes)), np.float32)
+    sp.symmetric_prepared(prep, 4, codes, len(codes), fp, mem=capi.TQ_MEM_HOST)
+    assert scale_rel_err(fp, GOLD[f"symfullprep_{tag}"], na, xns).max() <= FP32_TOL
+    X = np.ascontiguousarray(GOLD[f"X_{tag}"])
+    rot = np.zeros((len(X), sp.D), np.float32)
+    sp.rotated(X, len(X), rot, mem=capi.TQ_MEM_HOST)
+    np.testing.assert_array_equal(rot.view(np.uint32), GOLD[f"rot_{tag}"].view(np.uint32))
+    sp.close()
+
+
+@pytest.mark.parametrize("dim,bits", [(128, 4), (768, 4), (768, 8), (1000, 4), (2048, 3),
+                                      (4096, 4), (20, 5)])
+def test_random_full_light_against_oracle(gpu, dim, bits):
+    say(f"[gpu] sym qjl random dim={dim} bits={bits}")
+    capi = gpu
+    sp = capi.Space(dim, bits)
+    o = Oracle(dim, bits)
+    rng = np.random.default_rng(3 * dim + bits)
+    n = 700 if dim <= 1024 else 150
+    X = rng.standard_normal((n, dim)).astype(np.float32)
+    X[::7] *= 1e-4
+    codes = o.encode(X)
+    A = codes[:3]
+    na = o.trailer(A)[:, 0].astype(np.float64) ** 2
+    xns = o.trailer(codes)[:, 0].astype(np.float64) ** 2
+    L = olib()
+    for variant, fn in ((capi.TQ_SYM_FULL, L.tqo_distance_symmetric_full),
+                        (capi.TQ_SYM_LIGHT, L.tqo_distance_symmetric_light)):
+        got = host_sym(sp, A, codes, variant)
+        want = oracle_pairs(fn, o, A, codes)
+        err = scale_rel_err(got, want, na, xns)
+        say(f"  variant {variant}: max scale-rel err {err.max():.3e}")
+        assert err.max() <= FP32_TOL
+    rot = np.zeros((n, sp.D), np.float32)
+    sp.rotated(X, n, rot, mem=capi.TQ_MEM_HOST)
+    want = np.zeros((n, sp.D), np.float32)
+    L.tqo_rotated_coords_batch(o.h, ptr(X), n, ptr(want))
+    np.testing.assert_array_equal(rot.view(np.uint32), want.view(np.uint32))
+    sp.close()
+
+
+@pytest.mark.parametrize("dim,bits", [(128, 4), (768, 4), (768, 8), (1000, 4), (32, 2)])
+def test_symmetric_mfma_against_fp64(gpu, dim, bits):
+    say(f"[gpu] sym mfma dim={dim} bits={bits}")
+    capi = gpu
+    sp = capi.Space(dim, bits)
+    o = Oracle(dim, bits)
+    rng = np.random.default_rng(dim + 17 * bits)
+    X = rng.standard_normal((1500, dim)).astype(np.float32)
+    codes = o.encode(X)
+    A = codes[:70]
+    got = host_sym(sp, A, codes, capi.TQ_SYM_MSE, mode=capi.TQ_MODE_MFMA)
+    exact = o.symmetric_f64(A, codes)
+    na = o.trailer(A)[:, 0].astype(np.float64) ** 2
+    xns = o.trailer(codes)[:, 0].astype(np.float64) ** 2
+    err = scale_rel_err(got, exact, na, xns)
+    say(f"  max scale-rel err vs fp64 {err.max():.3e}")
+    assert err.max() <= MFMA_TOL
+    sp.close()
+
+
+def test_qjl_variants_reject_mfma_mode(gpu):
+    capi = gpu
+    sp = capi.Space(128, 4)
+    codes = np.zeros((2, sp.cs), np.uint8)
+    with pytest.raises(capi.TQError, match="fp32 only"):
+        host_sym(sp, codes, codes, capi.TQ_SYM_FULL, mode=capi.TQ_MODE_MFMA)
+    with pytest.raises(capi.TQError, match="unknown variant"):
+        host_sym(sp, codes, codes, 7)
+    sp.close()
+
+
+def test_dropin_non_hot_methods(gpu):
+    import turboquant
+    say("[gpu] drop-in non-hot methods")
+    dim, bits = 200, 4
+    s = turboquant.TurboQuantSpace(dim, bits_per_coord=bits)
+    o = Oracle(dim, bits)
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((300, dim)).astype(np.float32)
+    codes = s.encode_batch(X)
+    np.testing.assert_array_equal(codes, o.encode(X))
+    rot = s.rotated_coords_batch(X)
+    assert rot.shape == (300, s.padded_dim()) and rot.dtype == np.float32
+    want = np.zeros_like(rot)
+    olib().tqo_rotated_coords_batch(o.h, ptr(X), 300, ptr(want))
+    np.testing.assert_array_equal(rot.view(np.uint32), want.view(np.uint32))
+    A = codes[:5]
+    na = o.trailer(A)[:, 0].astype(np.float64) ** 2
+    xns = o.trailer(codes)[:, 0].astype(np.float64) ** 2
+    L = olib()
+    full = s.distance_m_to_n_symmetric_full(A, codes)
+    assert scale_rel_err(full, oracle_pairs(L.tqo_distance_symmetric_full, o, A, codes), na,
+                         xns).max() <= FP32_TOL
+    light = s.distance_m_to_n_symmetric_light(A, codes)
+    assert scale_rel_err(light, oracle_pairs(L.tqo_distance_symmetric_light, o, A, codes), na,
+                         xns).max() <= FP32_TOL
+    fullp = s.distance_m_to_n_symmetric_full_prepared(A, codes)
+    wantp = oracle_pairs(L.tqo_distance_symmetric_full_prepared, o, A, codes)
+    assert scale_rel_err(fullp, wantp, na, xns).max() <= FP32_TOL
+    pq = s.prepare_symmetric_query(codes[2])
+    assert isinstance(pq, turboquant.PreparedSymCode)
+    one = s.distance_1_to_n_symmetric_full(pq, codes)
+    np.testing.assert_array_equal(one, fullp[2])
+    assert s.distance_symmetric_full_prepared(pq, codes[7]) == one[7]
+    msym = s.distance_m_to_n_symmetric(A, codes, mode="mfma")
+    assert scale_rel_err(msym, o.symmetric_f64(A, codes), na, xns).max() <= MFMA_TOL
+    assert isinstance(s.prepare_query(X[0]), turboquant.PreparedQuery)
+    with pytest.raises(ValueError, match="must be uint8"):
+        s.prepare_symmetric_query(np.zeros(s.code_size_bytes(), np.float32))
+    with pytest.raises(ValueError, match="wrong byte size"):
+        s.prepare_symmetric_query(codes[0][:-1])
+    with pytest.raises(ValueError, match="wrong dim"):
+        s.rotated_coords_batch(np.zeros((3, dim + 1), np.float32))

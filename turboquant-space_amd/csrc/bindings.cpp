@@ -11,6 +11,7 @@
 #include <iostream>
 #include <string>
 #include <thread>
+#include <vector>
 
 #include "../../include/tq_hip.h"
 
@@ -66,6 +67,15 @@ int parse_mode(const std::string &mode) {
   if (mode == "mfma") return TQ_MODE_MFMA;
   throw py::value_error("mode must be 'fp32' or 'mfma'");
 }
+
+// Opaque prepared forms (bindings.cpp:56-57): host copies of the rows the
+// C ABI's prepare calls write.
+struct PreparedQuery {
+  std::vector<float> row;  // [q_rot | s_q | q_norm_sq | q_norm | 0 | 0]
+};
+struct PreparedSymCode {
+  std::vector<float> row;  // tq_prepare_symmetric's row
+};
 
 class Space {
  public:
@@ -174,7 +184,94 @@ class Space {
     return out;
   }
 
-  py::array_t<float> distance_m_to_n_symmetric(py::buffer codes_a, py::buffer codes_b) {
+  py::array_t<float> distance_m_to_n_symmetric(py::buffer codes_a, py::buffer codes_b,
+                                               const std::string &mode) {
+    return sym_matrix(codes_a, codes_b, TQ_SYM_MSE, parse_mode(mode));
+  }
+  // bindings.cpp:221-266
+  py::array_t<float> distance_m_to_n_symmetric_full(py::buffer codes_a, py::buffer codes_b) {
+    return sym_matrix(codes_a, codes_b, TQ_SYM_FULL, TQ_MODE_FP32);
+  }
+  py::array_t<float> distance_m_to_n_symmetric_light(py::buffer codes_a, py::buffer codes_b) {
+    return sym_matrix(codes_a, codes_b, TQ_SYM_LIGHT, TQ_MODE_FP32);
+  }
+
+  // bindings.cpp:119-132
+  py::array_t<float> rotated_coords_batch(py::buffer X) {
+    auto info = X.request();
+    ssize_t rows = 0;
+    const float *p = as_float_ptr(info, dim(), "input", 2, &rows);
+    py::array_t<float> out({rows, static_cast<ssize_t>(padded_dim())});
+    float *o = out.mutable_data();
+    run([&] { return tq_rotated_coords_batch(h_, p, (uint64_t)rows, o, TQ_MEM_HOST, nullptr); });
+    return out;
+  }
+
+  // bindings.cpp:268-274
+  PreparedQuery prepare_query(py::buffer query) {
+    auto qi = query.request();
+    const float *q = as_float_ptr(qi, dim(), "query", 1);
+    PreparedQuery pq;
+    pq.row.resize(info_.prepared_stride);
+    float *o = pq.row.data();
+    run([&] { return tq_prepare_queries(h_, q, 1, o, TQ_MEM_HOST, nullptr); });
+    return pq;
+  }
+
+  // bindings.cpp:276-282
+  PreparedSymCode prepare_symmetric_query(py::buffer code) {
+    auto ci = code.request();
+    const auto *c = static_cast<const uint8_t *>(as_bytes_ptr(ci, code_size(), "code"));
+    PreparedSymCode pq;
+    pq.row.resize(info_.sym_prepared_stride);
+    float *o = pq.row.data();
+    run([&] { return tq_prepare_symmetric(h_, c, 1, o, TQ_MEM_HOST, nullptr); });
+    return pq;
+  }
+
+  // bindings.cpp:284-291
+  float distance_symmetric_full_prepared(const PreparedSymCode &pq, py::buffer code_b) {
+    check_sym(pq);
+    auto ci = code_b.request();
+    const auto *c = static_cast<const uint8_t *>(as_bytes_ptr(ci, code_size(), "code_b"));
+    float d = 0.0f;
+    const float *r = pq.row.data();
+    run([&] { return tq_distance_symmetric_prepared(h_, r, 1, c, 1, &d, TQ_MEM_HOST, nullptr); });
+    return d;
+  }
+
+  // bindings.cpp:293-310
+  py::array_t<float> distance_1_to_n_symmetric_full(const PreparedSymCode &pq, py::buffer codes) {
+    check_sym(pq);
+    auto ci = codes.request();
+    const size_t n = code_count(ci);
+    py::array_t<float> out(static_cast<ssize_t>(n));
+    float *o = out.mutable_data();
+    const auto *c = static_cast<const uint8_t *>(ci.ptr);
+    const float *r = pq.row.data();
+    run([&] { return tq_distance_symmetric_prepared(h_, r, 1, c, n, o, TQ_MEM_HOST, nullptr); });
+    return out;
+  }
+
+  // bindings.cpp:312-334: each a code prepared once, then scored against all b.
+  py::array_t<float> distance_m_to_n_symmetric_full_prepared(py::buffer codes_a,
+                                                             py::buffer codes_b) {
+    return sym_matrix(codes_a, codes_b, TQ_SYM_FULL, TQ_MODE_FP32);
+  }
+
+ private:
+  // bindings.cpp:159-167: itemsize 1, total bytes a multiple of code size.
+  size_t code_count(const py::buffer_info &ci) const {
+    if (ci.itemsize != 1) throw py::value_error("codes must be uint8");
+    const ssize_t cs = static_cast<ssize_t>(code_size());
+    ssize_t total = total_of(ci);
+    if (total % cs != 0) throw py::value_error("codes size not a multiple of code_size_bytes");
+    return static_cast<size_t>(total / cs);
+  }
+
+  // bindings.cpp:199-211 (and the full/light variants): both inputs itemsize
+  // 1, totals multiples of the code size.
+  py::array_t<float> sym_matrix(py::buffer codes_a, py::buffer codes_b, int variant, int md) {
     auto ai = codes_a.request();
     auto bi = codes_b.request();
     if (ai.itemsize != 1 || bi.itemsize != 1) throw py::value_error("codes must be uint8");
@@ -187,18 +284,16 @@ class Space {
     float *o = out.mutable_data();
     const auto *a = static_cast<const uint8_t *>(ai.ptr);
     const auto *b = static_cast<const uint8_t *>(bi.ptr);
-    run([&] { return tq_distance_m_to_n_symmetric(h_, a, m, b, n, o, TQ_MEM_HOST, nullptr); });
+    run([&] {
+      return tq_distance_m_to_n_symmetric_ex(h_, a, m, b, n, o, variant, md, TQ_MEM_HOST,
+                                             nullptr);
+    });
     return out;
   }
 
- private:
-  // bindings.cpp:159-167: itemsize 1, total bytes a multiple of code size.
-  size_t code_count(const py::buffer_info &ci) const {
-    if (ci.itemsize != 1) throw py::value_error("codes must be uint8");
-    const ssize_t cs = static_cast<ssize_t>(code_size());
-    ssize_t total = total_of(ci);
-    if (total % cs != 0) throw py::value_error("codes size not a multiple of code_size_bytes");
-    return static_cast<size_t>(total / cs);
+  void check_sym(const PreparedSymCode &pq) const {
+    if (pq.row.size() != info_.sym_prepared_stride)
+      throw py::value_error("pq: prepared for a space of another dim");
   }
 
   template <class F>
@@ -220,6 +315,8 @@ class Space {
 
 PYBIND11_MODULE(_turboquant, m) {
   m.doc() = "TurboQuant encode + quantized distances on AMD MI355X (gfx950)";
+  py::class_<PreparedQuery>(m, "PreparedQuery");
+  py::class_<PreparedSymCode>(m, "PreparedSymCode");
   py::class_<Space>(m, "TurboQuantSpace")
       .def(py::init<size_t, int, uint64_t, uint64_t, int>(), py::arg("dim"),
            py::arg("bits_per_coord") = 4, py::arg("rot_seed") = 42ULL,
@@ -241,6 +338,19 @@ PYBIND11_MODULE(_turboquant, m) {
       .def("distance_m_to_n", &Space::distance_m_to_n, py::arg("queries"), py::arg("codes"),
            py::arg("mode") = "fp32")
       .def("distance_m_to_n_symmetric", &Space::distance_m_to_n_symmetric, py::arg("codes_a"),
-           py::arg("codes_b"));
+           py::arg("codes_b"), py::arg("mode") = "fp32")
+      .def("distance_m_to_n_symmetric_full", &Space::distance_m_to_n_symmetric_full,
+           py::arg("codes_a"), py::arg("codes_b"))
+      .def("distance_m_to_n_symmetric_light", &Space::distance_m_to_n_symmetric_light,
+           py::arg("codes_a"), py::arg("codes_b"))
+      .def("rotated_coords_batch", &Space::rotated_coords_batch, py::arg("X"))
+      .def("prepare_query", &Space::prepare_query, py::arg("query"))
+      .def("prepare_symmetric_query", &Space::prepare_symmetric_query, py::arg("code"))
+      .def("distance_symmetric_full_prepared", &Space::distance_symmetric_full_prepared,
+           py::arg("pq"), py::arg("code_b"))
+      .def("distance_1_to_n_symmetric_full", &Space::distance_1_to_n_symmetric_full,
+           py::arg("pq"), py::arg("codes"))
+      .def("distance_m_to_n_symmetric_full_prepared",
+           &Space::distance_m_to_n_symmetric_full_prepared, py::arg("codes_a"), py::arg("codes_b"));
   m.def("device_count", &tq_device_count);
 }

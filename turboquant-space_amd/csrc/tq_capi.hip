@@ -14,6 +14,7 @@
 
 #include "tq_internal.h"
 #include "tq_mfma.h"
+#include "tq_symfull.h"
 
 namespace tq {
 namespace capi {
@@ -162,6 +163,7 @@ int tq_space_create(uint64_t dim, int bits, uint64_t rot_seed, uint64_t qjl_seed
                   std::sqrt(static_cast<float>(D));
   s->info.padded = D != dim;
   s->info.prepared_stride = (uint64_t)tq::prep_stride((int)D);
+  s->info.sym_prepared_stride = (uint64_t)tq::sym_prep_stride((int)D);
   make_codebook(bits - 1, s->cent, s->bnd);
   s->rot = make_signs(D, rot_seed);
   s->qjl = make_signs(D, qjl_seed);
@@ -396,12 +398,18 @@ int tq_distance_1_to_n(tq_space s, const float *query, const uint8_t *codes, uin
   return tq_distance_m_to_n(s, query, 1, codes, n, out, TQ_MODE_FP32, mem, stream);
 }
 
-int tq_distance_m_to_n_symmetric(tq_space s, const uint8_t *codes_a, uint64_t m,
-                                 const uint8_t *codes_b, uint64_t n, float *out, int mem,
-                                 void *stream) {
+int tq_distance_m_to_n_symmetric_ex(tq_space s, const uint8_t *codes_a, uint64_t m,
+                                    const uint8_t *codes_b, uint64_t n, float *out, int variant,
+                                    int mode, int mem, void *stream) {
   const char *what = "tq_distance_m_to_n_symmetric";
   if (!s) return fail(TQ_EINVAL, "%s: NULL space", what);
   if (int r = check_mem(mem, what)) return r;
+  if (variant != TQ_SYM_MSE && variant != TQ_SYM_FULL && variant != TQ_SYM_LIGHT)
+    return fail(TQ_EINVAL, "%s: unknown variant %d", what, variant);
+  if (mode != TQ_MODE_FP32 && mode != TQ_MODE_MFMA)
+    return fail(TQ_EINVAL, "%s: unknown mode %d", what, mode);
+  if (mode == TQ_MODE_MFMA && variant != TQ_SYM_MSE)
+    return fail(TQ_EUNSUPPORTED, "%s: the QJL-corrected variants run in fp32 only", what);
   if (m == 0 || n == 0) return TQ_OK;
   if (!codes_a || !codes_b || !out) return fail(TQ_EINVAL, "%s: NULL buffer", what);
   if (int r = select_device(s, what)) return r;
@@ -412,21 +420,108 @@ int tq_distance_m_to_n_symmetric(tq_space s, const uint8_t *codes_a, uint64_t m,
   if (int r = stage_in(ab, mem, st, what)) return r;
   if (int r = stage_in(bb, mem, st, what)) return r;
   if (int r = stage_in(ob, mem, st, what)) return r;
+  const auto *da = static_cast<const uint8_t *>(ab.dev);
+  const auto *db = static_cast<const uint8_t *>(bb.dev);
+  float *dout = static_cast<float *>(ob.dev);
   DevBuf prep;
-  hipError_t e = prep.alloc(m * s->info.prepared_stride * sizeof(float));
-  if (e != hipSuccess) return hip_fail(e, what);
-  e = tq::launch_prepare_from_codes(s->dev, static_cast<const uint8_t *>(ab.dev), (long)m,
-                                    static_cast<float *>(prep.p), st);
-  if (e != hipSuccess) return hip_fail(e, what);
-  e = tq::launch_scan_fp32(s->dev, static_cast<const float *>(prep.p), (long)m,
-                           static_cast<const uint8_t *>(bb.dev), (long)n,
-                           static_cast<float *>(ob.dev), true, st);
+  hipError_t e;
+  if (variant == TQ_SYM_MSE) {
+    e = prep.alloc(m * s->info.prepared_stride * sizeof(float));
+    if (e != hipSuccess) return hip_fail(e, what);
+    auto *dp = static_cast<float *>(prep.p);
+    e = tq::launch_prepare_from_codes(s->dev, da, (long)m, dp, st);
+    if (e == hipSuccess) {
+      if (mode == TQ_MODE_MFMA) {
+        e = tq::launch_scan_mfma(s->dev, dp, (long)m, db, (long)n, dout, st);
+        if (e == hipErrorNotSupported)
+          return fail(TQ_EUNSUPPORTED, "%s: MFMA mode does not support padded dim %d", what,
+                      s->dev.D);
+      } else {
+        e = tq::launch_scan_fp32(s->dev, dp, (long)m, db, (long)n, dout, true, st);
+      }
+    }
+  } else {
+    e = prep.alloc(m * s->info.sym_prepared_stride * sizeof(float));
+    if (e != hipSuccess) return hip_fail(e, what);
+    auto *dp = static_cast<float *>(prep.p);
+    e = tq::launch_prepare_sym(s->dev, da, (long)m, variant == TQ_SYM_LIGHT, dp, st);
+    if (e == hipSuccess) e = tq::launch_sym_qjl(s->dev, dp, (long)m, db, (long)n, dout, st);
+  }
   if (e != hipSuccess) return hip_fail(e, what);
   if (mem == TQ_MEM_DEVICE) {
+    // `prep` is freed on return; make sure the scan that reads it is done.
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, what);
     return TQ_OK;
   }
+  return stage_out(ob, mem, st, what);
+}
+
+int tq_distance_m_to_n_symmetric(tq_space s, const uint8_t *codes_a, uint64_t m,
+                                 const uint8_t *codes_b, uint64_t n, float *out, int mem,
+                                 void *stream) {
+  return tq_distance_m_to_n_symmetric_ex(s, codes_a, m, codes_b, n, out, TQ_SYM_MSE,
+                                         TQ_MODE_FP32, mem, stream);
+}
+
+int tq_prepare_symmetric(tq_space s, const uint8_t *codes, uint64_t m, float *prepared, int mem,
+                         void *stream) {
+  const char *what = "tq_prepare_symmetric";
+  if (!s) return fail(TQ_EINVAL, "%s: NULL space", what);
+  if (int r = check_mem(mem, what)) return r;
+  if (m == 0) return TQ_OK;
+  if (!codes || !prepared) return fail(TQ_EINVAL, "%s: NULL buffer", what);
+  if (int r = select_device(s, what)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Staged cb{codes, m * s->info.code_size, false};
+  Staged ob{prepared, m * s->info.sym_prepared_stride * sizeof(float), true};
+  if (int r = stage_in(cb, mem, st, what)) return r;
+  if (int r = stage_in(ob, mem, st, what)) return r;
+  hipError_t e = tq::launch_prepare_sym(s->dev, static_cast<const uint8_t *>(cb.dev), (long)m,
+                                        false, static_cast<float *>(ob.dev), st);
+  if (e != hipSuccess) return hip_fail(e, what);
+  return stage_out(ob, mem, st, what);
+}
+
+int tq_distance_symmetric_prepared(tq_space s, const float *prepared, uint64_t m,
+                                   const uint8_t *codes, uint64_t n, float *out, int mem,
+                                   void *stream) {
+  const char *what = "tq_distance_symmetric_prepared";
+  if (!s) return fail(TQ_EINVAL, "%s: NULL space", what);
+  if (int r = check_mem(mem, what)) return r;
+  if (m == 0 || n == 0) return TQ_OK;
+  if (!prepared || !codes || !out) return fail(TQ_EINVAL, "%s: NULL buffer", what);
+  if (int r = select_device(s, what)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Staged pb{prepared, m * s->info.sym_prepared_stride * sizeof(float), false};
+  Staged cb{codes, n * s->info.code_size, false};
+  Staged ob{out, m * n * sizeof(float), true};
+  if (int r = stage_in(pb, mem, st, what)) return r;
+  if (int r = stage_in(cb, mem, st, what)) return r;
+  if (int r = stage_in(ob, mem, st, what)) return r;
+  hipError_t e = tq::launch_sym_qjl(s->dev, static_cast<const float *>(pb.dev), (long)m,
+                                    static_cast<const uint8_t *>(cb.dev), (long)n,
+                                    static_cast<float *>(ob.dev), st);
+  if (e != hipSuccess) return hip_fail(e, what);
+  return stage_out(ob, mem, st, what);
+}
+
+int tq_rotated_coords_batch(tq_space s, const float *X, uint64_t n, float *out, int mem,
+                            void *stream) {
+  const char *what = "tq_rotated_coords_batch";
+  if (!s) return fail(TQ_EINVAL, "%s: NULL space", what);
+  if (int r = check_mem(mem, what)) return r;
+  if (n == 0) return TQ_OK;
+  if (!X || !out) return fail(TQ_EINVAL, "%s: NULL buffer", what);
+  if (int r = select_device(s, what)) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Staged in{X, n * s->info.dim * sizeof(float), false};
+  Staged ob{out, n * s->info.padded_dim * sizeof(float), true};
+  if (int r = stage_in(in, mem, st, what)) return r;
+  if (int r = stage_in(ob, mem, st, what)) return r;
+  hipError_t e = tq::launch_rotated(s->dev, static_cast<const float *>(in.dev), (long)n,
+                                    static_cast<float *>(ob.dev), st);
+  if (e != hipSuccess) return hip_fail(e, what);
   return stage_out(ob, mem, st, what);
 }
 

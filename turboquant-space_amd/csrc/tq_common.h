@@ -47,6 +47,9 @@ hipError_t launch_prepare(const SpaceDev &s, const float *Q, long m,
                           float *prep, hipStream_t st);
 hipError_t launch_prepare_from_codes(const SpaceDev &s, const uint8_t *codes,
                                      long m, float *prep, hipStream_t st);
+// rotatedCoordsBatch: sigma-normalised rotated coordinates, [n][D] floats.
+hipError_t launch_rotated(const SpaceDev &s, const float *X, long n, float *out,
+                          hipStream_t st);
 // Plain fp32 scan: out[q*n + j] for q < m, j < n. `ws`, if given, holds
 // scan_fp32_ws_bytes(s, m) bytes of scratch; without it the scan allocates
 // (stream-ordered) what it needs.

@@ -28,6 +28,9 @@
 namespace tq {
 namespace {
 
+// What k_encode writes: codes, prepared queries, or rotated coordinates.
+constexpr int kModeEncode = 0, kModePrepare = 1, kModeRotated = 2;
+
 template <int TP>
 __device__ __forceinline__ void lane_butterfly(float (&r)[TP], int m, int lane) {
   const bool hi = (lane & m) != 0;
@@ -110,7 +113,7 @@ __device__ __forceinline__ int quantize_idx(float v, const float *bnd, int mbits
   return idx;
 }
 
-template <int G, int TP, bool NIB, bool PREP>
+template <int G, int TP, bool NIB, int MODE>
 __global__ __launch_bounds__(256) void k_encode(SpaceDev s, const float *__restrict__ X,
                                                 long n, uint8_t *__restrict__ codes,
                                                 float *__restrict__ prep) {
@@ -147,6 +150,7 @@ __global__ __launch_bounds__(256) void k_encode(SpaceDev s, const float *__restr
   // is vectorised by the reference build; prepareQuery (:443-446) adds into
   // the returned struct, which the compiler must assume aliases the query, so
   // it stays one scalar FMA chain (run by the group's first lane).
+  constexpr bool PREP = MODE == kModePrepare;
   float nsq;
   if constexpr (PREP) {
     float acc = 0.0f;
@@ -159,7 +163,10 @@ __global__ __launch_bounds__(256) void k_encode(SpaceDev s, const float *__restr
     if (active) nsq = tail_sumsq(nsq, x, q8 * 8, s.dim);
   }
   const float norm = sqrtf(nsq);
-  const float inv_norm = (norm > 1e-10f) ? (1.0f / norm) : 0.0f;
+  // rotatedCoordsBatch (:418-419) tests ||x||^2 and divides by its root;
+  // encodeVector and prepareQuery (:345-348, :446-447) test ||x||.
+  const float inv_norm = MODE == kModeRotated ? ((nsq > 1e-20f) ? (1.0f / sqrtf(nsq)) : 0.0f)
+                                              : ((norm > 1e-10f) ? (1.0f / norm) : 0.0f);
 
   // rotated = RHT_rot(x / ||x||), zero-padded (:346-350, turbo_quant.h:89-99).
 #pragma unroll
@@ -172,7 +179,18 @@ __global__ __launch_bounds__(256) void k_encode(SpaceDev s, const float *__restr
 #pragma unroll
   for (int t = 0; t < TP; ++t) r[t] = r[t] * s.inv_sqrt_D;
 
-  if constexpr (PREP) {
+  if constexpr (MODE == kModeRotated) {
+    // rotatedCoordsBatch (:411-435): the rotated coordinates times
+    // sqrt(D / sum r^2) (1 when the sum is below 1e-20), D floats per vector.
+    const float var = chain_sumsq<G, TP>(r, G / 8 * TP, h, j, lane, grp0);
+    const float inv_sigma = (var > 1e-20f) ? sqrtf((float)s.D / var) : 1.0f;
+    float *dst = prep + (active ? vec : 0L) * (long)s.D;
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < TP; ++t) dst[8 * (h * TP + t) + j] = r[t] * inv_sigma;
+    }
+    return;
+  } else if constexpr (PREP) {
     // TurboQuantPreparedQuery: q_rot, then s_q = RHT_qjl(q_rot) (:449-458).
     float *pq = prep + (active ? vec : 0L) * prep_stride(s.D);
     if (active) {
@@ -253,10 +271,11 @@ __global__ __launch_bounds__(256) void k_encode(SpaceDev s, const float *__restr
 
 // D == 4 (input dims 1..4): one thread per vector, the same arithmetic
 // written serially (the 8-wide chains are empty at this size).
-template <bool PREP>
+template <int MODE>
 __global__ __launch_bounds__(256) void k_encode_d4(SpaceDev s, const float *__restrict__ X,
                                                    long n, uint8_t *__restrict__ codes,
                                                    float *__restrict__ prep) {
+  constexpr bool PREP = MODE == kModePrepare;
   const long vec = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (vec >= n) return;
   const float *x = X + vec * s.dim;
@@ -269,7 +288,8 @@ __global__ __launch_bounds__(256) void k_encode_d4(SpaceDev s, const float *__re
     nsq = tail_sumsq(0.0f, x, 0, s.dim);
   }
   const float norm = sqrtf(nsq);
-  const float inv_norm = (norm > 1e-10f) ? (1.0f / norm) : 0.0f;
+  const float inv_norm = MODE == kModeRotated ? ((nsq > 1e-20f) ? (1.0f / sqrtf(nsq)) : 0.0f)
+                                              : ((norm > 1e-10f) ? (1.0f / norm) : 0.0f);
   for (int i = 0; i < 4; ++i) r[i] = (r[i] * inv_norm) * s.rot[i];
   auto wht4 = [&](float *v) {
     for (int st = 1; st < 4; st <<= 1)
@@ -282,7 +302,11 @@ __global__ __launch_bounds__(256) void k_encode_d4(SpaceDev s, const float *__re
     for (int i = 0; i < 4; ++i) v[i] = v[i] * s.inv_sqrt_D;
   };
   wht4(r);
-  if constexpr (PREP) {
+  if constexpr (MODE == kModeRotated) {
+    const float var = tail_sumsq(0.0f, r, 0, 4);
+    const float inv_sigma = (var > 1e-20f) ? sqrtf(4.0f / var) : 1.0f;
+    for (int i = 0; i < 4; ++i) prep[vec * 4 + i] = r[i] * inv_sigma;
+  } else if constexpr (PREP) {
     float *pq = prep + vec * prep_stride(4);
     for (int i = 0; i < 4; ++i) pq[i] = r[i];
     for (int i = 0; i < 4; ++i) r[i] = r[i] * s.qjl[i];
@@ -320,7 +344,7 @@ __global__ __launch_bounds__(256) void k_encode_d4(SpaceDev s, const float *__re
   }
 }
 
-template <int G, int TP, bool PREP>
+template <int G, int TP, int MODE>
 hipError_t launch_g(const SpaceDev &s, const float *X, long n, uint8_t *codes, float *prep,
                     hipStream_t st) {
   constexpr int threads = 256;
@@ -328,35 +352,35 @@ hipError_t launch_g(const SpaceDev &s, const float *X, long n, uint8_t *codes, f
   const long blocks = (n + per_block - 1) / per_block;
   const size_t lds = sizeof(float) * (2 * (size_t)s.D + 2 * (size_t)s.L);
   if (s.nib)
-    hipLaunchKernelGGL((k_encode<G, TP, true, PREP>), dim3((unsigned)blocks), dim3(threads), lds,
+    hipLaunchKernelGGL((k_encode<G, TP, true, MODE>), dim3((unsigned)blocks), dim3(threads), lds,
                        st, s, X, n, codes, prep);
   else
-    hipLaunchKernelGGL((k_encode<G, TP, false, PREP>), dim3((unsigned)blocks), dim3(threads), lds,
+    hipLaunchKernelGGL((k_encode<G, TP, false, MODE>), dim3((unsigned)blocks), dim3(threads), lds,
                        st, s, X, n, codes, prep);
   return hipGetLastError();
 }
 
-template <bool PREP>
+template <int MODE>
 hipError_t dispatch(const SpaceDev &s, const float *X, long n, uint8_t *codes, float *prep,
                     hipStream_t st) {
   if (n <= 0) return hipSuccess;
   switch (s.D) {
     case 4: {
       const long blocks = (n + 255) / 256;
-      hipLaunchKernelGGL((k_encode_d4<PREP>), dim3((unsigned)blocks), dim3(256), 0, st, s, X, n,
+      hipLaunchKernelGGL((k_encode_d4<MODE>), dim3((unsigned)blocks), dim3(256), 0, st, s, X, n,
                          codes, prep);
       return hipGetLastError();
     }
-    case 8: return launch_g<8, 1, PREP>(s, X, n, codes, prep, st);
-    case 16: return launch_g<8, 2, PREP>(s, X, n, codes, prep, st);
-    case 32: return launch_g<8, 4, PREP>(s, X, n, codes, prep, st);
-    case 64: return launch_g<8, 8, PREP>(s, X, n, codes, prep, st);
-    case 128: return launch_g<8, 16, PREP>(s, X, n, codes, prep, st);
-    case 256: return launch_g<8, 32, PREP>(s, X, n, codes, prep, st);
-    case 512: return launch_g<8, 64, PREP>(s, X, n, codes, prep, st);
-    case 1024: return launch_g<16, 64, PREP>(s, X, n, codes, prep, st);
-    case 2048: return launch_g<32, 64, PREP>(s, X, n, codes, prep, st);
-    case 4096: return launch_g<64, 64, PREP>(s, X, n, codes, prep, st);
+    case 8: return launch_g<8, 1, MODE>(s, X, n, codes, prep, st);
+    case 16: return launch_g<8, 2, MODE>(s, X, n, codes, prep, st);
+    case 32: return launch_g<8, 4, MODE>(s, X, n, codes, prep, st);
+    case 64: return launch_g<8, 8, MODE>(s, X, n, codes, prep, st);
+    case 128: return launch_g<8, 16, MODE>(s, X, n, codes, prep, st);
+    case 256: return launch_g<8, 32, MODE>(s, X, n, codes, prep, st);
+    case 512: return launch_g<8, 64, MODE>(s, X, n, codes, prep, st);
+    case 1024: return launch_g<16, 64, MODE>(s, X, n, codes, prep, st);
+    case 2048: return launch_g<32, 64, MODE>(s, X, n, codes, prep, st);
+    case 4096: return launch_g<64, 64, MODE>(s, X, n, codes, prep, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -367,12 +391,17 @@ bool encode_supported(int D) { return D >= 4 && D <= 4096; }
 
 hipError_t launch_encode(const SpaceDev &s, const float *X, long n, uint8_t *codes,
                          hipStream_t st) {
-  return dispatch<false>(s, X, n, codes, nullptr, st);
+  return dispatch<kModeEncode>(s, X, n, codes, nullptr, st);
 }
 
 hipError_t launch_prepare(const SpaceDev &s, const float *Q, long m, float *prep,
                           hipStream_t st) {
-  return dispatch<true>(s, Q, m, nullptr, prep, st);
+  return dispatch<kModePrepare>(s, Q, m, nullptr, prep, st);
+}
+
+hipError_t launch_rotated(const SpaceDev &s, const float *X, long n, float *out,
+                          hipStream_t st) {
+  return dispatch<kModeRotated>(s, X, n, nullptr, out, st);
 }
 
 }  // namespace tq

@@ -22,12 +22,16 @@ TQ_MEM_HOST = 0
 TQ_MEM_DEVICE = 1
 TQ_MODE_FP32 = 0
 TQ_MODE_MFMA = 1
+TQ_SYM_MSE = 0
+TQ_SYM_FULL = 1
+TQ_SYM_LIGHT = 2
 
 EXPORTED = [
     "tq_last_error", "tq_device_count", "tq_space_create", "tq_space_destroy",
     "tq_space_info_get", "tq_space_tables", "tq_encode_batch", "tq_prepare_queries",
     "tq_distance_1_to_n", "tq_distance_m_to_n", "tq_distance_prepared",
-    "tq_distance_m_to_n_symmetric", "tq_search_plan_create", "tq_search_plan_run",
+    "tq_distance_m_to_n_symmetric", "tq_distance_m_to_n_symmetric_ex", "tq_prepare_symmetric",
+    "tq_distance_symmetric_prepared", "tq_rotated_coords_batch", "tq_search_plan_create", "tq_search_plan_run",
     "tq_search_plan_run_graph", "tq_search_plan_tune", "tq_search_plan_stats",
     "tq_search_plan_destroy", "tq_search", "tq_topk_rows", "tq_server_create",
     "tq_server_destroy", "tq_server_add_collection", "tq_server_submit_append",
@@ -48,6 +52,7 @@ class SpaceInfo(ctypes.Structure):
         ("scale", ctypes.c_float),
         ("padded", ctypes.c_int32),
         ("prepared_stride", ctypes.c_uint64),
+        ("sym_prepared_stride", ctypes.c_uint64),
     ]
 
 
@@ -124,6 +129,14 @@ def lib():
         L.tq_distance_prepared.argtypes = [P, P, U64, P, U64, P, I, P]
         L.tq_distance_m_to_n_symmetric.restype = I
         L.tq_distance_m_to_n_symmetric.argtypes = [P, P, U64, P, U64, P, I, P]
+        L.tq_distance_m_to_n_symmetric_ex.restype = I
+        L.tq_distance_m_to_n_symmetric_ex.argtypes = [P, P, U64, P, U64, P, I, I, I, P]
+        L.tq_prepare_symmetric.restype = I
+        L.tq_prepare_symmetric.argtypes = [P, P, U64, P, I, P]
+        L.tq_distance_symmetric_prepared.restype = I
+        L.tq_distance_symmetric_prepared.argtypes = [P, P, U64, P, U64, P, I, P]
+        L.tq_rotated_coords_batch.restype = I
+        L.tq_rotated_coords_batch.argtypes = [P, P, U64, P, I, P]
         JP = ctypes.POINTER(SearchJob)
         TP = ctypes.POINTER(SearchTuning)
         L.tq_search_plan_create.restype = I
@@ -198,6 +211,7 @@ class Space:
         self.cs = int(info.code_size)
         self.bits = bits
         self.prep_stride = int(info.prepared_stride)
+        self.sym_prep_stride = int(info.sym_prepared_stride)
 
     def close(self):
         if self.h:
@@ -227,9 +241,20 @@ class Space:
     def one_to_n(self, q, codes, n, out, mem=TQ_MEM_DEVICE, stream=None):
         check(lib().tq_distance_1_to_n(self.h, ptr(q), ptr(codes), n, ptr(out), mem, stream))
 
-    def symmetric(self, A, m, B, n, out, mem=TQ_MEM_DEVICE, stream=None):
-        check(lib().tq_distance_m_to_n_symmetric(self.h, ptr(A), m, ptr(B), n, ptr(out), mem,
-                                                 stream))
+    def symmetric(self, A, m, B, n, out, mem=TQ_MEM_DEVICE, stream=None, variant=TQ_SYM_MSE,
+                  mode=TQ_MODE_FP32):
+        check(lib().tq_distance_m_to_n_symmetric_ex(self.h, ptr(A), m, ptr(B), n, ptr(out),
+                                                    variant, mode, mem, stream))
+
+    def prepare_symmetric(self, A, m, prep, mem=TQ_MEM_DEVICE, stream=None):
+        check(lib().tq_prepare_symmetric(self.h, ptr(A), m, ptr(prep), mem, stream))
+
+    def symmetric_prepared(self, prep, m, B, n, out, mem=TQ_MEM_DEVICE, stream=None):
+        check(lib().tq_distance_symmetric_prepared(self.h, ptr(prep), m, ptr(B), n, ptr(out), mem,
+                                                   stream))
+
+    def rotated(self, X, n, out, mem=TQ_MEM_DEVICE, stream=None):
+        check(lib().tq_rotated_coords_batch(self.h, ptr(X), n, ptr(out), mem, stream))
 
 
 def make_jobs(jobs):
