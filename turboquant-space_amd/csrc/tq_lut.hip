@@ -3,19 +3,18 @@
 // Same estimator as tq_scan.hip (distSearchScalarB4,
 // include/turboquant/space_turbo_quant.h:951-983; distBuildScalarB4 :985-1019
 // for the symmetric form), arranged so that a lane spends about two VALU
-// operations per coordinate and the scan runs at HBM speed:
+// operations per coordinate while the codes stream from HBM:
 //
-//  * Lanes own whole codes (64 codes per wavefront pass), so every lane of a
-//    wavefront works on the same coordinates at the same time and the query
-//    side is wavefront-uniform.
+//  * A few consecutive lanes (V) share one code, each scoring its own
+//    16-byte chunk (32 coordinates) of it, so one load instruction reads
+//    16*V contiguous bytes per code; the V partial sums meet through DPP.
 //  * MSE term through a pair table in LDS: for byte position p (coordinates
 //    2p, 2p+1) the 64 entries T_p[lo + 8*hi] = q[2p]*c[lo] + q[2p+1]*c[hi]
 //    cover every pair of 3-bit centroid indices. One lookup and one add per
 //    code byte. The byte offset 4*(lo + 8*hi) of an entry is obtained for all
 //    four bytes of a code word at once as (w & 0xEE..) + (w & 0x0E..) (no
 //    carries between bytes), and one v_perm per byte splices it under the
-//    table's base. All lanes read the same 64-entry table, which covers each
-//    LDS bank once: no bank conflicts.
+//    table's base.
 //  * QJL term as an exact integer dot product: s_q is rounded to a 15-bit
 //    fixed-point integer (scale 2^e per query) split into four signed 4-bit
 //    digits, and sum_{qjl bit set} S_i is four v_dot8 (i4 x i4 -> i32) per
@@ -85,156 +84,213 @@ __global__ __launch_bounds__(256) void k_qjl_limbs(SpaceDev s, const float *__re
   }
 }
 
-// Measured on MI355X at dim 768 / 4-bit over 10M codes: 512 threads, one
-// code per lane per block and one block of prefetch beat 1024 threads
-// (register spills / L1 thrash), two codes per lane and deeper prefetch.
-constexpr int kLutThreads = 512;
-constexpr int kLutWaves = kLutThreads / 64;
-constexpr int kLutCodesPerLane = 1;
-constexpr int kLutPrefetch = 1;  // blocks in flight ahead of the one computed
-
-// Work split of one block: SL coordinate slices x G code groups = 8
-// wavefronts. Wavefront w takes slice w % SL (WPW consecutive 32-bit words of
-// every code) of the CPL*64 codes of group w / SL; lane l owns codes
-// (group*CPL + c)*64 + l. One thread per code reduces the slices. The 8 wavefronts together read each block's
-// codes as one contiguous run of bytes, so every cache line is used whole
-// while it is still in L2.
+// Launch shape per padded dim D (threads, lanes per code V, register ring
+// slots R; R - 1 blocks are in flight while one is computed). V consecutive
+// lanes read 16*V contiguous bytes of one code, so one 16-byte load
+// instruction touches 64/V codes' cache lines instead of 64. Measured on
+// MI355X (10M codes of 524 B, loads only): 64 lanes on 64 codes stream at
+// 3.2-3.9 TB/s (L1 tag lookups), 4 lanes per code 5.5 TB/s, 8 lanes 6.0 TB/s,
+// fully coalesced 6.4 TB/s. With the table lookups added, V = 2 and 16
+// wavefronts per workgroup (4 per SIMD; at D = 1024 the pair tables fill
+// 128 KB of LDS, so one workgroup per CU) were fastest over D = 32..1024
+// (1-to-N over 10-20M codes: 4.2-4.9 TB/s), except 8 wavefronts at D = 128.
 template <int D>
-struct LutShape {
-  static constexpr int NW = D / 8;                      // words per code
-  static constexpr int SL = NW / 4 < 8 ? NW / 4 : 8;    // slices (>= 16 bytes each)
-  static constexpr int G = kLutWaves / SL;              // code groups
-  static constexpr int WPW = NW / SL;                   // words per slice
-  static constexpr int V4 = WPW / 4;                    // 16-byte loads per slice
-  static constexpr int CPL = G * kLutCodesPerLane * 64 <= kLutThreads ? kLutCodesPerLane : 1;  // codes per lane
-  static constexpr int CODES = G * CPL * 64;            // codes per block (<= 512)
+struct LutCfg {
+  static constexpr int THREADS = D == 128 ? 512 : 1024;
+  static constexpr int V = 2;
+  static constexpr int R = THREADS >= 1024 ? 2 : 3;
 };
 
+// Work split of one block: SL coordinate slices x G code groups = 8
+// wavefronts. A slice is SW = 4V consecutive 32-bit words (16V bytes) of a
+// code. Wavefront w takes slice w % SL of the 64 codes of group w / SL in V
+// load rounds: in round i, lane l loads 16-byte chunk l % V of the slice of
+// code i*64/V + l/V and scores it against that chunk's tables (a lane's
+// chunk, and so its tables and digit words, is the same in every round).
+// The V lanes of a code add their partial sums with DPP; the SL slice sums
+// of a code meet in LDS; one thread per code applies the epilogue.
+template <int D>
+struct LutShape {
+  static constexpr int THREADS = LutCfg<D>::THREADS;
+  static constexpr int WAVES = THREADS / 64;
+  static constexpr int NW = D / 8;  // 32-bit words of packed indices per code
+  static constexpr int V = LutCfg<D>::V < NW / 4 ? LutCfg<D>::V : NW / 4;
+  static constexpr int SW = 4 * V;                   // words per slice
+  static constexpr int SL = NW / SW;                 // slices per code
+  static constexpr int G = WAVES / SL;               // code groups
+  static constexpr int CODES = G * 64;               // codes per block
+  static_assert(SL >= 1 && SL <= WAVES && G * SL == WAVES, "every wavefront owns a (slice, group)");
+  static_assert(V == 1 || V == 2 || V == 4 || V == 8, "lanes per code");
+  static constexpr size_t lds_bytes = sizeof(float) * 64 * (D / 2) +  // pair tables
+                                      2 * 2 * 4 * (size_t)SL * CODES +  // slice partials
+                                      sizeof(int32_t) * (D / 2);        // QJL digit words
+  static_assert(lds_bytes <= 160 * 1024, "LDS of one CU");
+};
+
+// Sum over the V consecutive lanes that share a code (all of them get it).
+template <int V>
+__device__ __forceinline__ int group_sum_i(int v) {
+  if constexpr (V >= 2) v += __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);   // quad_perm [1,0,3,2]
+  if constexpr (V >= 4) v += __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);   // quad_perm [2,3,0,1]
+  if constexpr (V >= 8) v += __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true);  // row_half_mirror
+  return v;
+}
+template <int V>
+__device__ __forceinline__ float group_sum_f(float v) {
+  if constexpr (V >= 2) v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+  if constexpr (V >= 4) v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
+  if constexpr (V >= 8) v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, true));
+  return v;
+}
+
 template <bool SYM, int D>
-__global__ __launch_bounds__(kLutThreads) void k_lut_nib(SpaceDev s, const float *__restrict__ prep,
+__global__ __launch_bounds__(LutShape<D>::THREADS) void k_lut_nib(SpaceDev s, const float *__restrict__ prep,
                                                          const int32_t *__restrict__ ql, long m,
                                                          const uint8_t *__restrict__ codes, long n,
                                                          float *__restrict__ out) {
   using Sh = LutShape<D>;
-  constexpr int SL = Sh::SL, WPW = Sh::WPW, V4 = Sh::V4, CODES = Sh::CODES, kLutCPL = Sh::CPL;
-  static_assert(CODES <= kLutThreads, "one reducing thread per code");
-  static_assert(kLutWaves % SL == 0 && Sh::G * SL == kLutWaves, "every wavefront owns a (slice, group)");
+  constexpr int V = Sh::V, SW = Sh::SW, SL = Sh::SL, CODES = Sh::CODES;
+  constexpr int R = LutCfg<D>::R, THREADS = Sh::THREADS;
   // (D/2) tables of 64 floats; fixed-size arrays keep LDS addresses
   // compile-time constants. part_*: per-slice partial sums, double-buffered.
+  // ql_lds: the query's QJL digit words.
   __shared__ float tab[(D / 2) * 64];
   __shared__ float part_m[2][SL][CODES];
   __shared__ int part_q[2][SL][CODES];
+  __shared__ __align__(16) int32_t ql_lds[D / 2];
 
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int sl = wave % SL, grp = wave / SL;
+  const int chunk = lane % V;
+  const bool reducer = wave < Sh::G;  // wavefront-uniform: threads tid < CODES finish codes
   const long pstride = prep_stride(s.D);
   const long lstride = qlimb_stride(s.D);
   const long nblocks = (n + CODES - 1) / CODES;
+  const long G = gridDim.x;
+  const int w0 = sl * SW + 4 * chunk;  // this lane's first word in every code
+  // Byte address of the first table of this lane's chunk (a multiple of 256,
+  // so v_perm can splice an entry offset into its low byte).
+  const uint32_t tbase = (uint32_t)(w0 * 1024);
   float cent[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) cent[k] = s.cent[k < s.L ? k : 0];
 
-  auto load_block = [&](long b, uint4 (&r)[kLutCPL][V4]) {
-#pragma unroll
-    for (int c = 0; c < kLutCPL; ++c) {
-      const long code = b * CODES + (grp * kLutCPL + c) * 64 + lane;
-      const uint32_t *src = reinterpret_cast<const uint32_t *>(
-          codes + (code < n ? code : 0L) * (long)s.cs + sl * WPW * 4);
-#pragma unroll
-      for (int v = 0; v < V4; ++v)
-        r[c][v] = make_uint4(src[4 * v], src[4 * v + 1], src[4 * v + 2], src[4 * v + 3]);
-    }
-  };
-
   for (long qi = 0; qi < m; ++qi) {
     const float *pq = prep + qi * pstride;
+    const int32_t *L = ql + qi * lstride;
     __syncthreads();  // the previous query is done with the tables
-    for (int e = tid; e < (D / 2) * 64; e += kLutThreads) {
+    for (int e = tid; e < (D / 2) * 64; e += THREADS) {
       const int p = e >> 6, lo = e & 7, hi = (e >> 3) & 7;
       tab[e] = fmaf(pq[2 * p + 1], cent[hi], pq[2 * p] * cent[lo]);
     }
+    if constexpr (!SYM)
+      for (int e = tid; e < D / 2; e += THREADS) ql_lds[e] = L[4 + e];
     __syncthreads();
     const float qns = pq[2 * s.D];
     const float qn = pq[2 * s.D + 1];
-    const int32_t *L = ql + qi * lstride;
     const int qe = SYM ? 0 : L[0];
     const int qsum = SYM ? 0 : L[1];
     float *orow = out + qi * n;
+    // This lane's digit words (4 words x 4 digits), in registers for the scan.
+    int32_t lw[SYM ? 1 : 4][4];
+    if constexpr (!SYM) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int4 v = *reinterpret_cast<const int4 *>(&ql_lds[4 * (w0 + u)]);
+        lw[u][0] = v.x;
+        lw[u][1] = v.y;
+        lw[u][2] = v.z;
+        lw[u][3] = v.w;
+      }
+    }
 
-    // Blocks b, b + grid, ... stream through a ring of kLutPrefetch register
-    // buffers: the loads of a block are issued kLutPrefetch blocks before use.
-    uint4 ring[kLutPrefetch][kLutCPL][V4];
-    long b = blockIdx.x;
-#pragma unroll
-    for (int d = 0; d < kLutPrefetch; ++d)
-      if (b + (long)d * gridDim.x < nblocks) load_block(b + (long)d * gridDim.x, ring[d]);
-    int buf = 0;
-    for (; b < nblocks; b += gridDim.x, buf ^= 1) {
-      uint4 cur[kLutCPL][V4];
-#pragma unroll
-      for (int c = 0; c < kLutCPL; ++c)
-#pragma unroll
-        for (int v = 0; v < V4; ++v) cur[c][v] = ring[0][c][v];
-#pragma unroll
-      for (int d = 0; d + 1 < kLutPrefetch; ++d)
-#pragma unroll
-        for (int c = 0; c < kLutCPL; ++c)
-#pragma unroll
-          for (int v = 0; v < V4; ++v) ring[d][c][v] = ring[d + 1][c][v];
-      if (b + (long)kLutPrefetch * gridDim.x < nblocks)
-        load_block(b + (long)kLutPrefetch * gridDim.x, ring[kLutPrefetch - 1]);
-      // The reducing thread's trailer, fetched early.
-      const long rcode = b * CODES + tid;
-      const bool ract = tid < CODES && rcode < n;
-      float xn = 0.0f, gamma = 0.0f, sigma = 0.0f;
-      if (ract) {
-        const float *meta = reinterpret_cast<const float *>(codes + rcode * (long)s.cs + s.packed);
-        xn = meta[0];
-        gamma = meta[1];
-        sigma = meta[2];
+    // Ring of register buffers: slot k holds a block's chunks (V rounds) and
+    // the trailers [norm, gamma, sigma] of the codes this thread finishes.
+    // Every load is unconditional (block index clamped to the last block,
+    // code index to the last code), trailers first, so the in-order load
+    // counter lets the newer blocks stay in flight while the oldest is used.
+    uint4 ring[R][V];
+    float tx[R], tg[R], tsg[R];
+    auto load = [&](auto slot, long blk) {
+      constexpr int k = decltype(slot)::value;
+      const long bb = blk < nblocks ? blk : nblocks - 1;
+      __builtin_amdgcn_sched_barrier(0);
+      {
+        // Reducing wavefronts take their codes' trailers; the others read one
+        // shared trailer (a single cache line), so every wavefront issues the
+        // same loads. D >= 32: code sizes D/2 + 12 are multiples of 4.
+        long rc = bb * CODES + tid;
+        rc = rc < n ? rc : n - 1;
+        const float *mp = reinterpret_cast<const float *>(codes + (reducer ? rc : 0L) * (long)s.cs + s.packed);
+        tx[k] = mp[0];
+        tg[k] = mp[1];
+        tsg[k] = mp[2];
       }
 #pragma unroll
-      for (int c = 0; c < kLutCPL; ++c) {
+      for (int i = 0; i < V; ++i) {
+        long code = bb * CODES + grp * 64 + i * (64 / V) + lane / V;
+        code = code < n ? code : n - 1;
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(codes + code * (long)s.cs + 4 * w0);
+        ring[k][i] = make_uint4(src[0], src[1], src[2], src[3]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    };
+
+    long b = blockIdx.x;
+    asm volatile("" : "+s"(b));  // keeps the first blocks' loads inside the query loop
+    int buf = 0;
+    // One block from ring slot `slot`: partial sums, slot reloaded with block
+    // b + R*G, barrier, the reducing threads finish block b.
+    auto step = [&](auto slot) {
+      constexpr int k = decltype(slot)::value;
+#pragma unroll
+      for (int i = 0; i < V; ++i) {
         float am0 = 0.0f, am1 = 0.0f;
         int aq0 = 0, aq1 = 0, aq2 = 0, aq3 = 0;
 #pragma unroll
-        for (int k2 = 0; k2 < WPW; ++k2) {
-          const uint4 q4 = cur[c][k2 >> 2];
-          const uint32_t w = (k2 & 3) == 0 ? q4.x : (k2 & 3) == 1 ? q4.y : (k2 & 3) == 2 ? q4.z : q4.w;
-          const int wi = sl * WPW + k2;
+        for (int u = 0; u < 4; ++u) {
+          const uint4 q4 = ring[k][i];
+          const uint32_t w = u == 0 ? q4.x : u == 1 ? q4.y : u == 2 ? q4.z : q4.w;
           const uint32_t m2 = (w & 0xEEEEEEEEu) + (w & 0x0E0E0E0Eu);
-          const uint32_t tb = (uint32_t)(wi * 1024);  // byte offset of table 4*wi
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const uint32_t addr =
-                __builtin_amdgcn_perm(tb + 256u * k, m2, 0x07060500u | (uint32_t)k);
-            const float v = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(tab) + addr);
-            if (k & 1)
+          for (int kk = 0; kk < 4; ++kk) {
+            const uint32_t addr = __builtin_amdgcn_perm(tbase, m2, 0x07060500u | (uint32_t)kk);
+            const float v = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(tab) + addr +
+                                                             1024 * u + 256 * kk);
+            if (kk & 1)
               am1 += v;
             else
               am0 += v;
           }
           if constexpr (!SYM) {
             const int t = (int)(w & 0x11111111u);
-            aq0 = __builtin_amdgcn_sdot8(t, L[4 + 4 * wi + 0], aq0, false);
-            aq1 = __builtin_amdgcn_sdot8(t, L[4 + 4 * wi + 1], aq1, false);
-            aq2 = __builtin_amdgcn_sdot8(t, L[4 + 4 * wi + 2], aq2, false);
-            aq3 = __builtin_amdgcn_sdot8(t, L[4 + 4 * wi + 3], aq3, false);
+            aq0 = __builtin_amdgcn_sdot8(t, lw[u][0], aq0, false);
+            aq1 = __builtin_amdgcn_sdot8(t, lw[u][1], aq1, false);
+            aq2 = __builtin_amdgcn_sdot8(t, lw[u][2], aq2, false);
+            aq3 = __builtin_amdgcn_sdot8(t, lw[u][3], aq3, false);
           }
         }
-        const int slot = (grp * kLutCPL + c) * 64 + lane;
-        part_m[buf][sl][slot] = am0 + am1;
-        part_q[buf][sl][slot] = aq0 + 16 * aq1 + 256 * aq2 + 4096 * aq3;
+        const float am = group_sum_f<V>(am0 + am1);
+        const int aq = SYM ? 0 : group_sum_i<V>(aq0 + 16 * aq1 + 256 * aq2 + 4096 * aq3);
+        if (chunk == i % V) {
+          const int slot_i = grp * 64 + i * (64 / V) + lane / V;
+          part_m[buf][sl][slot_i] = am;
+          part_q[buf][sl][slot_i] = aq;
+        }
       }
-      __syncthreads();
-      if (ract) {
+      const float xn = tx[k], gamma = tg[k], sigma = tsg[k];
+      load(slot, b + (long)R * G);
+      __syncthreads();  // partials of block b complete
+      if (reducer) {
+        const long rcode = b * CODES + tid;
         float am = 0.0f;
         int T = 0;
 #pragma unroll
-        for (int k = 0; k < SL; ++k) {
-          am += part_m[buf][k][tid];
-          T += part_q[buf][k][tid];
+        for (int j = 0; j < SL; ++j) {
+          am += part_m[buf][j][tid];
+          T += part_q[buf][j][tid];
         }
         const float ip_mse = am * sigma;
         float corr = 0.0f;
@@ -244,7 +300,39 @@ __global__ __launch_bounds__(kLutThreads) void k_lut_nib(SpaceDev s, const float
         }
         const float ip = (ip_mse + corr) * xn * qn;
         const float d = qns + xn * xn - 2.0f * ip;
-        orow[rcode] = d > 0.0f ? d : 0.0f;
+        if (rcode < n) orow[rcode] = d > 0.0f ? d : 0.0f;
+      }
+      b += G;
+      buf ^= 1;
+    };
+
+    static_assert(R == 2 || R == 3, "the unrolled ring below is written for two or three slots");
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, 1>;
+    using S2 = std::integral_constant<int, R == 3 ? 2 : 0>;
+    if (b < nblocks) {
+      // Blocks of this workgroup: b, b + G, ... < nblocks. The loop runs
+      // whole turns of the ring and has no exit mid-turn, so the load
+      // counter sees the same order of pending loads on every entry to its
+      // header; the blocks of a last partial turn follow it.
+      const long nb = (nblocks - 1 - b) / G + 1;
+      load(S0{}, b);
+      load(S1{}, b + G);
+      if constexpr (R == 3) {
+        load(S2{}, b + 2 * G);
+        for (long t = nb / 3; t > 0; --t) {
+          step(S0{});
+          step(S1{});
+          step(S2{});
+        }
+        if (nb % 3 >= 1) step(S0{});
+        if (nb % 3 >= 2) step(S1{});
+      } else {
+        for (long t = nb / 2; t > 0; --t) {
+          step(S0{});
+          step(S1{});
+        }
+        if (nb % 2) step(S0{});
       }
     }
   }
@@ -271,14 +359,13 @@ namespace {
 template <bool SYM, int D>
 hipError_t launch_lut_d(const SpaceDev &s, const float *prep, const int32_t *ql, long m,
                         const uint8_t *codes, long n, float *out, hipStream_t st) {
-  constexpr size_t lds =
-      sizeof(float) * 64 * (D / 2) + 2 * 2 * 4 * LutShape<D>::SL * LutShape<D>::CODES;
+  constexpr size_t lds = LutShape<D>::lds_bytes;
   const long per_cu = (long)((160 * 1024) / lds) < 4 ? (long)((160 * 1024) / lds) : 4;
   long blocks = (long)cu_count() * (per_cu > 0 ? per_cu : 1);
   const long need = (n + LutShape<D>::CODES - 1) / LutShape<D>::CODES;
   if (blocks > need) blocks = need;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL((k_lut_nib<SYM, D>), dim3((unsigned)blocks), dim3(kLutThreads), 0, st, s,
+  hipLaunchKernelGGL((k_lut_nib<SYM, D>), dim3((unsigned)blocks), dim3(LutShape<D>::THREADS), 0, st, s,
                      prep, ql, m, codes, n, out);
   return hipGetLastError();
 }
