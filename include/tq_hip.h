@@ -122,8 +122,8 @@ int tq_distance_m_to_n_symmetric(tq_space s, const uint8_t *codes_a,
 
 /* Symmetric estimator variants. */
 #define TQ_SYM_MSE 0   /* distBuildScalar(B4)           space_turbo_quant.h:688-714, :985-1019 */
-#define TQ_SYM_FULL 1  /* distBuildFullScalar           :731-811 */
-#define TQ_SYM_LIGHT 2 /* distBuildLightScalar          :893-945 */
+#define TQ_SYM_FULL 1  /* distBuildFullScalar           :731-799 */
+#define TQ_SYM_LIGHT 2 /* distBuildLightScalar          :902-945 */
 
 /* distanceBatchMToNSymmetric / ...SymmetricFull / ...SymmetricLight
  * (:602-647; bindings.cpp:197-266). mode TQ_MODE_MFMA is available for
@@ -134,7 +134,7 @@ int tq_distance_m_to_n_symmetric_ex(tq_space s, const uint8_t *codes_a,
                                     uint64_t n, float *out, int variant,
                                     int mode, int mem, void *stream);
 
-/* prepareSymmetricQuery -- space_turbo_quant.h:498-532 (bindings.cpp:276-282).
+/* prepareSymmetricQuery -- space_turbo_quant.h:499-532 (bindings.cpp:276-282).
  * m rows of sym_prepared_stride floats: the a side of the full estimator
  * (reconstruction, transposed-Hadamard signs, Hadamard of the reconstruction
  * with the sign term folded in, norm^2, norm). */
