@@ -7,18 +7,20 @@
 // followed by a per-pair fp32 epilogue with the code's norm/gamma/sigma and
 // the query's norm (identical to the plain path's).
 //
-// Precision: fp16 holds neither the queries nor the Lloyd-Max centroids to
-// fp32 accuracy, so both are split into hi + lo fp16 parts (v = hi + lo with
-// hi = fp16(v), lo = fp16(v - hi), ~22 significant bits) and the MSE term is
-// accumulated as q_hi*c_hi + q_lo*c_hi + q_hi*c_lo (the dropped q_lo*c_lo is
-// ~2^-22 relative). The +-1 QJL factors are exact in fp16 and the QJL term
-// enters the distance with a factor scale*gamma ~ 1e-2, so s_q is used at
-// fp16 (one pass; its rounding moves distances by ~1e-7 of their scale).
-// Everything accumulates in fp32 (v_mfma_f32_32x32x16_f16).
-// Queries are scaled by 2^10 and centroids by 2^8 before the split so the lo
-// parts stay in fp16's normal range; the epilogue removes the exact powers of
-// two. Measured error against the fp64 restatement of the estimator is held
-// below 1e-4 of (||q||^2 + ||x||^2).
+// Precision: fp16 does not hold the queries to fp32 accuracy, so they are
+// split into hi + lo fp16 parts (v = hi + lo with hi = fp16(v),
+// lo = fp16(v - hi), ~22 significant bits) and the MSE term is accumulated as
+// q_hi*c + q_lo*c. For nibble codes the space's centroid scale makes every
+// scaled centroid nearly exact in fp16 (within 2^-15; SpaceDev::cscale), so
+// c is one fp16 value; byte codes (up to 128 centroids) split c as well and
+// add q_hi*c_lo (the dropped q_lo*c_lo is ~2^-22 relative). The +-1 QJL
+// factors are exact in fp16 and the QJL term enters the distance with a
+// factor scale*gamma ~ 1e-2, so s_q is used at fp16 (one pass; its rounding
+// moves distances by ~1e-7 of their scale). Everything accumulates in fp32
+// (v_mfma_f32_32x32x16_f16). Queries are scaled by 2^10 and centroids by
+// cscale (2^8 for byte codes) so the lo parts stay in fp16's normal range;
+// the epilogue removes the scales. Measured error against the fp64
+// restatement of the estimator is held below 1e-4 of (||q||^2 + ||x||^2).
 //
 // Tiling: a workgroup (4 wavefronts) owns 64 queries x 256 codes; each
 // wavefront owns 64 queries x 64 codes as 2 x 2 blocks of 32 x 32 MFMA
@@ -81,7 +83,7 @@ struct MfmaShape {
 
 // Two workgroups per CU (two wavefronts per SIMD) so one wavefront's MFMA
 // chain overlaps the other's decode, LDS traffic and barrier waits.
-template <bool NIB>
+template <bool NIB, bool LO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma(SpaceDev s, const _Float16 *__restrict__ planes,
                                               const float *__restrict__ prep, long m,
                                               const uint8_t *__restrict__ codes, long n,
@@ -102,10 +104,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     l_qn[tid][0] = pq[2 * s.D];
     l_qn[tid][1] = pq[2 * s.D + 1];
   }
+  static_assert(NIB || LO, "byte codes carry c_lo");
   for (int i = tid; i < 256; i += blockDim.x) {
-    const float c = i < s.L ? s.cent[i] * kCScale : 0.0f;
-    const _Float16 ch = (_Float16)c;
-    const _Float16 cl = (_Float16)(c - (float)ch);
+    _Float16 ch, cl;
+    cent_h(s, i, ch, cl);
     l_tab[i] = h_bits(ch) | (h_bits(cl) << 16);
   }
 
@@ -114,9 +116,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     uint32_t bytes[4][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const float c = k < s.L ? s.cent[k] * kCScale : 0.0f;
-      const _Float16 chh = (_Float16)c;
-      const _Float16 cll = (_Float16)(c - (float)chh);
+      _Float16 chh, cll;
+      cent_h(s, k, chh, cll);
       const uint32_t hb = h_bits(chh), lb = h_bits(cll);
       const int word = k >> 2, sh = 8 * (k & 3);
       bytes[0][word] |= (hb & 0xFFu) << sh;
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       for (int c = 0; c < 2; ++c) {
         const int code = wave * 64 + 32 * c + lr;
         if constexpr (NIB) {
-          decode_nib_word(c_lds[b][(code * CROW + kb * 8 + 4 * kg) / 4], ptab, bh[c], bl[c], bt[c]);
+          decode_nib_word<LO>(c_lds[b][(code * CROW + kb * 8 + 4 * kg) / 4], ptab, bh[c], bl[c], bt[c]);
         } else {
           uint32_t u[8];
           const uint32_t w0 = c_lds[b][(code * CROW + kb * 16 + 8 * kg) / 4];
@@ -247,7 +248,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       TQ_PASS(acc_m, 0, bh)
       TQ_PASS(acc_q, 2, bt)
       TQ_PASS(acc_m, 1, bh)
-      TQ_PASS(acc_m, 0, bl)
+      if constexpr (LO) {
+        TQ_PASS(acc_m, 0, bl)
+      }
 #undef TQ_PASS
     }
     if (ch + 1 < nchunks) store_chunk(b ^ 1);
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (j >= n) continue;
     const float *meta = reinterpret_cast<const float *>(codes + j * (long)s.cs + s.packed);
     const float xn = meta[0], gamma = meta[1], sigma = meta[2];
-    const float fm = sigma * kMseUnscale;
+    const float fm = sigma * s.mse_unscale;
     const float fq = s.scale * gamma * kQjlUnscale;
     const float xns = xn * xn;
 #pragma unroll
@@ -316,10 +319,12 @@ hipError_t launch_scan_mfma(const SpaceDev &s, const float *prep, long m, const 
     e = hipGetLastError();
     if (e != hipSuccess) break;
     dim3 grid((unsigned)((n + kTN - 1) / kTN), (unsigned)((mc + kTQ - 1) / kTQ));
-    if (s.nib)
-      hipLaunchKernelGGL(k_mfma<true>, grid, dim3(256), 0, st, s, planes, pc, mc, codes, n, oc);
+    if (s.nib && s.mfma_lo)
+      hipLaunchKernelGGL((k_mfma<true, true>), grid, dim3(256), 0, st, s, planes, pc, mc, codes, n, oc);
+    else if (s.nib)
+      hipLaunchKernelGGL((k_mfma<true, false>), grid, dim3(256), 0, st, s, planes, pc, mc, codes, n, oc);
     else
-      hipLaunchKernelGGL(k_mfma<false>, grid, dim3(256), 0, st, s, planes, pc, mc, codes, n, oc);
+      hipLaunchKernelGGL((k_mfma<false, true>), grid, dim3(256), 0, st, s, planes, pc, mc, codes, n, oc);
     e = hipGetLastError();
   }
   hipError_t f = hipFreeAsync(planes, st);

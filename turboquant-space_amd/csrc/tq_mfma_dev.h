@@ -10,10 +10,17 @@ namespace mfma {
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 
+// Queries (q_rot, s_q) are scaled by 2^10; centroids by the space's cscale
+// (SpaceDev::cscale, undone with SpaceDev::mse_unscale).
 constexpr float kQScale = 1024.0f;              // 2^10 on q_rot and s_q
-constexpr float kCScale = 256.0f;               // 2^8 on the centroids
-constexpr float kMseUnscale = 1.0f / 262144.0f; // 2^-18
 constexpr float kQjlUnscale = 1.0f / 1024.0f;   // 2^-10
+
+// fp16 hi and lo parts of the scaled centroid k (0 past the L centroids).
+__device__ __forceinline__ void cent_h(const SpaceDev &s, int k, _Float16 &ch, _Float16 &cl) {
+  const float c = k < s.L ? s.cent[k] * s.cscale : 0.0f;
+  ch = (_Float16)c;
+  cl = (_Float16)(c - (float)ch);
+}
 
 __device__ __forceinline__ uint32_t h_bits(_Float16 h) {
   return (uint32_t)__builtin_bit_cast(uint16_t, h);
@@ -30,7 +37,9 @@ struct PermTab {
 
 // 8 nibble units in one code word -> B fragments in the order
 // (0, 2, 4, 6, 1, 3, 5, 7): bytes of the even (odd) coordinates' values are
-// looked up four at a time, then interleaved into fp16 pairs.
+// looked up four at a time, then interleaved into fp16 pairs. The c_lo
+// fragment is decoded only for spaces that need it (SpaceDev::mfma_lo).
+template <bool LO>
 __device__ __forceinline__ void decode_nib_word(uint32_t w, const PermTab &t, h8 &bh, h8 &bl,
                                                 h8 &bt) {
   const uint32_t ie = (w >> 1) & 0x07070707u;  // indices of coordinates 0, 2, 4, 6
@@ -47,7 +56,7 @@ __device__ __forceinline__ void decode_nib_word(uint32_t w, const PermTab &t, h8
     r[3] = __builtin_amdgcn_perm(Ho, Lo, 0x07030602u);
     bh = __builtin_bit_cast(h8, r);
   }
-  {
+  if constexpr (LO) {
     const uint32_t Le = __builtin_amdgcn_perm(t.ll_hi, t.ll_lo, ie);
     const uint32_t He = __builtin_amdgcn_perm(t.lh_hi, t.lh_lo, ie);
     const uint32_t Lo = __builtin_amdgcn_perm(t.ll_hi, t.ll_lo, io);

@@ -122,10 +122,12 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
     const int nchunks = s.D / kSKC;
     const int cparts = nib ? 1 : 2;  // 16-byte loads per code and chunk
 
+    // c_lo pass only where the space's scaled centroids are not exact in
+    // fp16 (all byte codes; nibble codes normally not).
+    const bool lo = !nib || s.mfma_lo != 0;
     for (int i = tid; i < 256; i += blockDim.x) {
-      const float c = i < s.L ? s.cent[i] * kCScale : 0.0f;
-      const _Float16 ch = (_Float16)c;
-      const _Float16 cl = (_Float16)(c - (float)ch);
+      _Float16 ch, cl;
+      cent_h(s, i, ch, cl);
       l_tab[i] = h_bits(ch) | (h_bits(cl) << 16);
     }
     if (tid < QT) {
@@ -139,9 +141,8 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
       uint32_t bytes[4][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}};
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const float c = k < s.L ? s.cent[k] * kCScale : 0.0f;
-        const _Float16 chh = (_Float16)c;
-        const _Float16 cll = (_Float16)(c - (float)chh);
+        _Float16 chh, cll;
+        cent_h(s, k, chh, cll);
         const uint32_t hb = h_bits(chh), lb = h_bits(cll);
         const int word = k >> 2, sh = 8 * (k & 3);
         bytes[0][word] |= (hb & 0xFFu) << sh;
@@ -255,8 +256,11 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
           for (int c = 0; c < 2; ++c) {
             const int code = wave * 64 + 32 * c + lr;
             if (nib) {
-              decode_nib_word(c_lds[b][(code * kSCRow + kb * 8 + 4 * kg) / 4], ptab, bh[c], bl[c],
-                              bt[c]);
+              const uint32_t w = c_lds[b][(code * kSCRow + kb * 8 + 4 * kg) / 4];
+              if (lo)
+                decode_nib_word<true>(w, ptab, bh[c], bl[c], bt[c]);
+              else
+                decode_nib_word<false>(w, ptab, bh[c], bl[c], bt[c]);
             } else {
               const uint32_t w0 = c_lds[b][(code * kSCRow + kb * 16 + 8 * kg) / 4];
               const uint32_t w1 = c_lds[b][(code * kSCRow + kb * 16 + 8 * kg) / 4 + 1];
@@ -276,7 +280,9 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
           TQ_PASS(acc_m, 0, bh)
           TQ_PASS(acc_q, 2, bt)
           TQ_PASS(acc_m, 1, bh)
-          TQ_PASS(acc_m, 0, bl)
+          if (lo) {
+            TQ_PASS(acc_m, 0, bl)
+          }
 #undef TQ_PASS
         }
         if (ch + 1 < nchunks) store_chunk(b ^ 1);
@@ -293,7 +299,7 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
         const float *meta =
             reinterpret_cast<const float *>(job.codes + (j < n ? j : 0L) * (long)s.cs + s.packed);
         const float xn = meta[0], gamma = meta[1], sigma = meta[2];
-        fm[c] = sigma * kMseUnscale;
+        fm[c] = sigma * s.mse_unscale;
         fq[c] = s.scale * gamma * kQjlUnscale;
         xnv[c] = xn;
         xns[c] = xn * xn;
