@@ -73,6 +73,15 @@ constexpr int kTN = 256;           // codes per workgroup (64 per wavefront)
 constexpr int kKC = 32;            // coordinates per chunk
 constexpr int kARow = kKC + 8;     // padded A row, in halves (80 B)
 constexpr int kPlanes = 3;         // q_hi, q_lo, s_hi
+// Chunks whose loads are in flight: 3 for nibble codes, 2 for byte codes
+// (twice the code bytes per chunk, and the register budget of two
+// wavefronts per SIMD).
+template <bool NIB>
+constexpr int ring_slots() { return NIB ? 3 : 2; }
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// The same at 4-byte alignment (code bytes: codes are 4-byte aligned; the
+// padded LDS code rows are too).
+typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
 
 template <bool NIB>
 struct MfmaShape {
@@ -138,9 +147,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const int nchunks = s.D / kKC;
 
   // Per-thread load sources and LDS destinations, fixed for the whole walk
-  // over the padded dim; a chunk only adds its coordinate offset.
+  // over the padded dim; a chunk only adds its coordinate offset. Rows of
+  // queries past m and codes past n read row 0 / code 0 instead: every load
+  // is unconditional, and those rows and columns of the accumulators are
+  // never written out (an MFMA output element depends only on its own A row
+  // and B column).
   const _Float16 *asrc[ALOADS];
-  bool aok[ALOADS];
   int adst[ALOADS];
 #pragma unroll
   for (int i = 0; i < ALOADS; ++i) {
@@ -149,51 +161,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int row = (idx / (kKC / 8)) % kTQ;
     const int part = idx % (kKC / 8);
     const long q = q0 + row;
-    aok[i] = q < m;
-    asrc[i] = planes + p * mD + (aok[i] ? q : 0L) * s.D + 8 * part;
+    asrc[i] = planes + p * mD + (q < m ? q : 0L) * s.D + 8 * part;
     adst[i] = (p * kTQ + row) * kARow + 8 * part;
   }
   const uint8_t *csrc[CLOADS];
-  bool cok[CLOADS];
   int cdst[CLOADS];
 #pragma unroll
   for (int i = 0; i < CLOADS; ++i) {
     const int idx = i * 256 + tid;
     const int c = idx / CPARTS, part = idx % CPARTS;
     const long j = j0 + c;
-    cok[i] = j < n;
-    csrc[i] = codes + (cok[i] ? j : 0L) * (long)s.cs + 16 * part;
+    csrc[i] = codes + (j < n ? j : 0L) * (long)s.cs + 16 * part;
     cdst[i] = (c * CROW + 16 * part) / 4;
   }
 
-  uint4 ra[ALOADS], rc[CLOADS];
-  auto load_chunk = [&](int ch) {
-    const int k0 = ch * kKC;
+  // Register ring: slot k holds the query-plane and code loads of one chunk
+  // (a group). Groups are issued three chunks before they are stored to LDS,
+  // so the HBM latency of the code bytes is covered by three chunks of
+  // matrix work; the hardware's load counter completes in order, so a group
+  // is always issued before every group needed after it, and the store of a
+  // chunk waits for its own group only.
+  constexpr int kRing = ring_slots<NIB>();
+  u32x4 ring_a[kRing][ALOADS], ring_c[kRing][CLOADS];
+  auto load_group = [&](auto slot, int ch) __attribute__((always_inline)) {
+    constexpr int k = decltype(slot)::value;
+    const int cc = ch < nchunks ? ch : nchunks - 1;
+    const int k0 = cc * kKC;
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int i = 0; i < ALOADS; ++i)
-      ra[i] = aok[i] ? *reinterpret_cast<const uint4 *>(asrc[i] + k0) : make_uint4(0u, 0u, 0u, 0u);
+    for (int i = 0; i < ALOADS; ++i) ring_a[k][i] = *reinterpret_cast<const u32x4 *>(asrc[i] + k0);
 #pragma unroll
-    for (int i = 0; i < CLOADS; ++i) {
-      if (cok[i]) {
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(csrc[i] + (NIB ? k0 / 2 : k0));
-        rc[i] = make_uint4(src[0], src[1], src[2], src[3]);
-      } else {
-        rc[i] = make_uint4(0u, 0u, 0u, 0u);
-      }
-    }
+    for (int i = 0; i < CLOADS; ++i)
+      ring_c[k][i] = *reinterpret_cast<const u32x4a4 *>(csrc[i] + (NIB ? k0 / 2 : k0));
+    __builtin_amdgcn_sched_barrier(0);
   };
-  auto store_chunk = [&](int b) {
+  auto store_group = [&](auto slot, int b) __attribute__((always_inline)) {
+    constexpr int k = decltype(slot)::value;
     _Float16 *abuf = &a_lds[b][0][0][0];
 #pragma unroll
-    for (int i = 0; i < ALOADS; ++i) *reinterpret_cast<uint4 *>(abuf + adst[i]) = ra[i];
+    for (int i = 0; i < ALOADS; ++i) *reinterpret_cast<u32x4 *>(abuf + adst[i]) = ring_a[k][i];
 #pragma unroll
-    for (int i = 0; i < CLOADS; ++i) {
-      uint32_t *dst = &c_lds[b][cdst[i]];
-      dst[0] = rc[i].x;
-      dst[1] = rc[i].y;
-      dst[2] = rc[i].z;
-      dst[3] = rc[i].w;
-    }
+    for (int i = 0; i < CLOADS; ++i) *reinterpret_cast<u32x4a4 *>(&c_lds[b][cdst[i]]) = ring_c[k][i];
   };
 
   f16v acc_m[2][2], acc_q[2][2];
@@ -207,12 +215,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         acc_q[r][c][v] = 0.0f;
       }
 
-  load_chunk(0);
-  store_chunk(0);
-  __syncthreads();
-  for (int ch = 0; ch < nchunks; ++ch) {
+  // One chunk: the matrix work on LDS buffer ch & 1, then chunk ch + 1 from
+  // ring slot `slot` into the other buffer, the slot reloaded with chunk
+  // ch + 1 + kRing, one barrier.
+  int ch = 0;
+  // Inlined at each use: the ring arrays must stay in registers.
+  auto step = [&](auto slot) __attribute__((always_inline)) {
     const int b = ch & 1;
-    if (ch + 1 < nchunks) load_chunk(ch + 1);
 #pragma unroll
     for (int kb = 0; kb < kKC / 16; ++kb) {
       h8 a[2][kPlanes];
@@ -253,8 +262,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       }
 #undef TQ_PASS
     }
-    if (ch + 1 < nchunks) store_chunk(b ^ 1);
+    store_group(slot, b ^ 1);
+    load_group(slot, ch + 1 + kRing);
     __syncthreads();
+    ++ch;
+  };
+
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  using S2 = std::integral_constant<int, kRing == 3 ? 2 : 0>;
+  load_group(S0{}, 0);
+  load_group(S1{}, 1);
+  if constexpr (kRing == 3) load_group(S2{}, 2);
+  store_group(S0{}, 0);
+  load_group(S0{}, kRing);
+  __syncthreads();
+  // Whole turns of the ring (no exit mid-turn), then the chunks of a last
+  // partial turn. Chunks past the end are reloads of the last one, never
+  // stored.
+  if constexpr (kRing == 3) {
+    for (int t = nchunks / 3; t > 0; --t) {
+      step(S1{});
+      step(S2{});
+      step(S0{});
+    }
+    if (nchunks % 3 >= 1) step(S1{});
+    if (nchunks % 3 >= 2) step(S2{});
+  } else {
+    for (int t = nchunks / 2; t > 0; --t) {
+      step(S1{});
+      step(S0{});
+    }
+    if (nchunks % 2) step(S1{});
   }
 
   // Epilogue: output element v of a 32x32 block sits at row
