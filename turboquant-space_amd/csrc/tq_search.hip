@@ -225,116 +225,123 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
         }
       };
 
-      f16v acc_m[QR][2], acc_q[QR][2];
+      // The tile's contraction and its distances into the lists, with the
+      // c_lo pass compiled in or out (a branch around MFMAs inside the
+      // unrolled loop would split its schedule).
+      auto scan_tile = [&](auto lo_c) __attribute__((always_inline)) {
+        constexpr bool LO = decltype(lo_c)::value;
+        f16v acc_m[QR][2], acc_q[QR][2];
 #pragma unroll
-      for (int r = 0; r < QR; ++r)
+        for (int r = 0; r < QR; ++r)
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+          for (int c = 0; c < 2; ++c)
 #pragma unroll
-          for (int v = 0; v < 16; ++v) {
-            acc_m[r][c][v] = 0.0f;
-            acc_q[r][c][v] = 0.0f;
-          }
+            for (int v = 0; v < 16; ++v) {
+              acc_m[r][c][v] = 0.0f;
+              acc_q[r][c][v] = 0.0f;
+            }
 
-      __syncthreads();  // the previous tile's transpose reads are done
-      load_chunk(0);
-      store_chunk(0);
-      __syncthreads();
-      for (int ch = 0; ch < nchunks; ++ch) {
-        const int b = ch & 1;
-        if (ch + 1 < nchunks) load_chunk(ch + 1);
+        __syncthreads();  // the previous tile's transpose reads are done
+        load_chunk(0);
+        store_chunk(0);
+        __syncthreads();
+        for (int ch = 0; ch < nchunks; ++ch) {
+          const int b = ch & 1;
+          if (ch + 1 < nchunks) load_chunk(ch + 1);
 #pragma unroll
-        for (int kb = 0; kb < kSKC / 16; ++kb) {
-          h8 a[QR][kSPlanes];
+          for (int kb = 0; kb < kSKC / 16; ++kb) {
+            h8 a[QR][kSPlanes];
 #pragma unroll
-          for (int r = 0; r < QR; ++r)
+            for (int r = 0; r < QR; ++r)
 #pragma unroll
-            for (int p = 0; p < kSPlanes; ++p)
-              a[r][p] = *reinterpret_cast<const h8 *>(&a_lds[b][p][32 * r + lr][kb * 16 + 8 * kg]);
-          h8 bh[2], bl[2], bt[2];
+              for (int p = 0; p < kSPlanes; ++p)
+                a[r][p] = *reinterpret_cast<const h8 *>(&a_lds[b][p][32 * r + lr][kb * 16 + 8 * kg]);
+            h8 bh[2], bl[2], bt[2];
 #pragma unroll
-          for (int c = 0; c < 2; ++c) {
-            const int code = wave * 64 + 32 * c + lr;
-            if (nib) {
-              const uint32_t w = c_lds[b][(code * kSCRow + kb * 8 + 4 * kg) / 4];
-              if (lo)
-                decode_nib_word<true>(w, ptab, bh[c], bl[c], bt[c]);
-              else
-                decode_nib_word<false>(w, ptab, bh[c], bl[c], bt[c]);
-            } else {
-              const uint32_t w0 = c_lds[b][(code * kSCRow + kb * 16 + 8 * kg) / 4];
-              const uint32_t w1 = c_lds[b][(code * kSCRow + kb * 16 + 8 * kg) / 4 + 1];
-              uint32_t u[8];
+            for (int c = 0; c < 2; ++c) {
+              const int code = wave * 64 + 32 * c + lr;
+              if (nib) {
+                const uint32_t w = c_lds[b][(code * kSCRow + kb * 8 + 4 * kg) / 4];
+                decode_nib_word<LO>(w, ptab, bh[c], bl[c], bt[c]);
+              } else {
+                const uint32_t w0 = c_lds[b][(code * kSCRow + kb * 16 + 8 * kg) / 4];
+                const uint32_t w1 = c_lds[b][(code * kSCRow + kb * 16 + 8 * kg) / 4 + 1];
+                uint32_t u[8];
 #pragma unroll
-              for (int v = 0; v < 4; ++v) {
-                u[v] = (w0 >> (8 * v)) & 0xFFu;
-                u[4 + v] = (w1 >> (8 * v)) & 0xFFu;
+                for (int v = 0; v < 4; ++v) {
+                  u[v] = (w0 >> (8 * v)) & 0xFFu;
+                  u[4 + v] = (w1 >> (8 * v)) & 0xFFu;
+                }
+                decode8(u, l_tab, bh[c], bl[c], bt[c]);
               }
-              decode8(u, l_tab, bh[c], bl[c], bt[c]);
+            }
+#define TQ_PASS(ACC, A, B)                                                               \
+    _Pragma("unroll") for (int r = 0; r < QR; ++r)                                         \
+        _Pragma("unroll") for (int c = 0; c < 2; ++c) ACC[r][c] =                          \
+            __builtin_amdgcn_mfma_f32_32x32x16_f16(a[r][A], B[c], ACC[r][c], 0, 0, 0);
+            TQ_PASS(acc_m, 0, bh)
+            TQ_PASS(acc_q, 2, bt)
+            TQ_PASS(acc_m, 1, bh)
+            if constexpr (LO) {
+              TQ_PASS(acc_m, 0, bl)
+            }
+#undef TQ_PASS
+          }
+          if (ch + 1 < nchunks) store_chunk(b ^ 1);
+          __syncthreads();
+        }
+
+        // Distances (the reference epilogue, as in k_mfma) into the lists, one
+        // 32-query block at a time through the transpose area.
+        const long jbase = j0 + wave * 64;
+        float fm[2], fq[2], xnv[2], xns[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const long j = jbase + 32 * c + lr;
+          const float *meta =
+              reinterpret_cast<const float *>(job.codes + (j < n ? j : 0L) * (long)s.cs + s.packed);
+          const float xn = meta[0], gamma = meta[1], sigma = meta[2];
+          fm[c] = sigma * s.mse_unscale;
+          fq[c] = s.scale * gamma * kQjlUnscale;
+          xnv[c] = xn;
+          xns[c] = xn * xn;
+        }
+#pragma unroll
+        for (int r = 0; r < QR; ++r) {
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+              const int row = 8 * (v >> 2) + 4 * kg + (v & 3);
+              const int qrow = 32 * r + row;
+              const float ip = (acc_m[r][c][v] * fm[c] + acc_q[r][c][v] * fq[c]) * xnv[c] * l_qn[qrow][1];
+              const float d = l_qn[qrow][0] + xns[c] - 2.0f * ip;
+              t_lds[wave][row][32 * c + lr] = d > 0.0f ? d : 0.0f;
+            }
+          __syncthreads();
+          // QR = 2: lanes 32r .. 32r+31 own this block's queries and read all
+          // 64 codes. QR = 1: lane q and q + 32 read codes 0-31 and 32-63 of
+          // query q. Codes past n and lanes that own nothing here offer the
+          // empty entry (inf, ~0), which no list takes.
+          const int h = QR == 1 ? (lane >> 5) : 0;
+          const bool mine = QR == 1 || (lane >> 5) == r;
+          const float *rowp = &t_lds[wave][lane & 31][32 * h];
+          const long cbase = jbase + 32 * h;
+          if (jbase < n) {
+            for (int k = 0; k < 64 / (3 - QR); ++k) {
+              const bool ok = mine && cbase + k < n;
+              const float d = ok ? rowp[k] : __builtin_inff();
+              const uint32_t id = ok ? (uint32_t)(cbase + k) : 0xFFFFFFFFu;
+              if (__any(lex_less(d, id, td[K - 1], ti[K - 1]))) topk_insert<K>(td, ti, d, id);
             }
           }
-#define TQ_PASS(ACC, A, B)                                                               \
-  _Pragma("unroll") for (int r = 0; r < QR; ++r)                                         \
-      _Pragma("unroll") for (int c = 0; c < 2; ++c) ACC[r][c] =                          \
-          __builtin_amdgcn_mfma_f32_32x32x16_f16(a[r][A], B[c], ACC[r][c], 0, 0, 0);
-          TQ_PASS(acc_m, 0, bh)
-          TQ_PASS(acc_q, 2, bt)
-          TQ_PASS(acc_m, 1, bh)
-          if (lo) {
-            TQ_PASS(acc_m, 0, bl)
-          }
-#undef TQ_PASS
+          __syncthreads();
         }
-        if (ch + 1 < nchunks) store_chunk(b ^ 1);
-        __syncthreads();
-      }
-
-      // Distances (the reference epilogue, as in k_mfma) into the lists, one
-      // 32-query block at a time through the transpose area.
-      const long jbase = j0 + wave * 64;
-      float fm[2], fq[2], xnv[2], xns[2];
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const long j = jbase + 32 * c + lr;
-        const float *meta =
-            reinterpret_cast<const float *>(job.codes + (j < n ? j : 0L) * (long)s.cs + s.packed);
-        const float xn = meta[0], gamma = meta[1], sigma = meta[2];
-        fm[c] = sigma * s.mse_unscale;
-        fq[c] = s.scale * gamma * kQjlUnscale;
-        xnv[c] = xn;
-        xns[c] = xn * xn;
-      }
-#pragma unroll
-      for (int r = 0; r < QR; ++r) {
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-#pragma unroll
-          for (int v = 0; v < 16; ++v) {
-            const int row = 8 * (v >> 2) + 4 * kg + (v & 3);
-            const int qrow = 32 * r + row;
-            const float ip = (acc_m[r][c][v] * fm[c] + acc_q[r][c][v] * fq[c]) * xnv[c] * l_qn[qrow][1];
-            const float d = l_qn[qrow][0] + xns[c] - 2.0f * ip;
-            t_lds[wave][row][32 * c + lr] = d > 0.0f ? d : 0.0f;
-          }
-        __syncthreads();
-        // QR = 2: lanes 32r .. 32r+31 own this block's queries and read all
-        // 64 codes. QR = 1: lane q and q + 32 read codes 0-31 and 32-63 of
-        // query q. Codes past n and lanes that own nothing here offer the
-        // empty entry (inf, ~0), which no list takes.
-        const int h = QR == 1 ? (lane >> 5) : 0;
-        const bool mine = QR == 1 || (lane >> 5) == r;
-        const float *rowp = &t_lds[wave][lane & 31][32 * h];
-        const long cbase = jbase + 32 * h;
-        if (jbase < n) {
-          for (int k = 0; k < 64 / (3 - QR); ++k) {
-            const bool ok = mine && cbase + k < n;
-            const float d = ok ? rowp[k] : __builtin_inff();
-            const uint32_t id = ok ? (uint32_t)(cbase + k) : 0xFFFFFFFFu;
-            if (__any(lex_less(d, id, td[K - 1], ti[K - 1]))) topk_insert<K>(td, ti, d, id);
-          }
-        }
-        __syncthreads();
-      }
+      };
+      if (lo)
+        scan_tile(std::true_type{});
+      else
+        scan_tile(std::false_type{});
     }
 
     // This wavefront's lists for the item.
