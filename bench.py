@@ -275,6 +275,51 @@ def bench_serving(torch, capi, sh, qps_by_mode, seconds, n_per_coll=SERVE_N, see
             "bits": BITS, "k": 10, "modes": results}
 
 
+RECALL_N = 1_000_000
+RECALL_Q = 256
+
+
+def bench_recall(torch, capi, sh, n=RECALL_N, m=RECALL_Q, k=10):
+    """Recall@1 and recall@10 of the quantized searches against the exact
+    neighbours (squared L2 over the raw vectors, fp64 brute force on the GPU)
+    on a sample of the bench workload: n vectors and m queries of the same
+    recipe (dim 768, 8,192-centre Gaussian mixture, std 0.3, L2-normalised;
+    seeds 40, 41). The reference reports the same quantity
+    (python/benchmarks/run_benchmark.py:251-278)."""
+    from turboquant.recall import exact_topk, recall_at_k
+    sp = capi.Space(DIM, BITS)
+    X = torch.empty((n, DIM), device="cuda")
+    codes = torch.empty((n, sp.cs), dtype=torch.uint8, device="cuda")
+    centres = None
+    for start, x, centres in gaussian_mixture(torch, n, DIM, 8192, 0.3, 40):
+        X[start:start + x.shape[0]] = x
+        sp.encode(x, x.shape[0], codes[start:start + x.shape[0]], stream=sh)
+    _, Q, _ = next(gaussian_mixture(torch, m, DIM, 8192, 0.3, 41, centres=centres))
+    torch.cuda.synchronize()
+    gt = exact_topk(Q, X, k).cpu().numpy()
+    del X
+    od = torch.empty((m, k), device="cuda")
+    oi = torch.empty((m, k), dtype=torch.int64, device="cuda")
+    plan = capi.SearchPlan([(sp, Q, m, codes, n, od, oi)], k)
+    plan.run(stream=sh)
+    torch.cuda.synchronize()
+    ids_mfma = oi.cpu().numpy()
+    plan.close()
+    dist = torch.empty((m, n), device="cuda")
+    sp.m_to_n(Q, m, codes, n, dist, mode=capi.TQ_MODE_FP32, stream=sh)
+    capi.topk_rows(dist, m, n, k, od, oi, stream=sh)
+    torch.cuda.synchronize()
+    ids_fp32 = oi.cpu().numpy()
+    del dist, codes
+    torch.cuda.empty_cache()
+    res = {"vectors": n, "queries": m, "k": k, "dim": DIM, "bits": BITS}
+    for name, ids in (("mfma_fused_search", ids_mfma), ("fp32_distances_topk", ids_fp32)):
+        res[name] = {"recall_at_1": recall_at_k(ids, gt, 1), "recall_at_10": recall_at_k(ids, gt, k)}
+    log(f"recall@10 vs exact: fused MFMA search {res['mfma_fused_search']['recall_at_10']:.4f}, "
+        f"fp32 path {res['fp32_distances_topk']['recall_at_10']:.4f}")
+    return res
+
+
 def cpu_baseline(codes_host, Q_host, budget_s=8.0):
     """The reference CPU build (oracle/_ref, compiled in place from the
     reference sources with its Release flags) on this host's cores:
@@ -325,6 +370,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched uneven-launch measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-recall", action="store_true", help="skip the recall measurement")
     ap.add_argument("--serve-seconds", type=float, default=12.0)
     ap.add_argument("--qps-mfma", type=float, default=40.0)
     ap.add_argument("--qps-fp32", type=float, default=4.0)
@@ -438,6 +484,7 @@ def main():
     # before the batched and serving runs build their own.
     del codes, out, one
     torch.cuda.empty_cache()
+    recall = None if args.no_recall else bench_recall(torch, capi, sh)
     batched = None if args.no_batched else bench_batched(torch, capi, stream, sh)
     serving = None
     if not args.no_serving:
@@ -459,6 +506,7 @@ def main():
         "mfma_path_dist_per_s": results.get("mfma", {}).get("dist_per_s"),
         "distance_1_to_n": one_to_n,
         "encode_vec_per_s": enc_vps,
+        "recall": recall,
         "batched_uneven": batched,
         "serving": serving,
     }
