@@ -22,7 +22,7 @@
 // what makes the reference's reduction order cheap to reproduce: the chain
 // runs in registers, the 8-way fold is three cross-lane adds. Butterfly
 // steps 1, 2, 4 cross lanes j, steps 8..8*TP/2 stay in registers, larger
-// steps cross lanes h.
+// steps cross lanes h (DPP within rows of 16 lanes, shuffles beyond).
 #include "tq_common.h"
 
 namespace tq {
@@ -31,13 +31,43 @@ namespace {
 // What k_encode writes: codes, prepared queries, or rotated coordinates.
 constexpr int kModeEncode = 0, kModePrepare = 1, kModeRotated = 2;
 
-template <int TP>
-__device__ __forceinline__ void lane_butterfly(float (&r)[TP], int m, int lane) {
-  const bool hi = (lane & m) != 0;
+// Value of `v` in another lane through DPP (no LDS): CTRL 0xB1 / 0x4E =
+// quad_perm lane^1 / lane^2; 0x104 = row_shl:4 (lane+4 within a row of 16);
+// 0x114 = row_shr:4 (lane-4); 0x128 = row_ror:8 (lane^8 within a row);
+// 0x141 = row_half_mirror (7-i within each 8 lanes). Lanes whose source is
+// outside their row read 0.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+
+// One butterfly step across lanes lane and lane^M: the lane with the lower
+// index keeps a + b, the other a - b (a = lower lane's value). Steps 1, 2, 4
+// and 8 stay inside a row of 16 lanes and go through DPP; wider ones use a
+// cross-lane shuffle.
+template <int TP, int M>
+__device__ __forceinline__ void lane_butterfly(float (&r)[TP], int lane) {
+  const bool hi = (lane & M) != 0;
 #pragma unroll
   for (int t = 0; t < TP; ++t) {
-    float p = __shfl_xor(r[t], m);
-    r[t] = hi ? (p - r[t]) : (r[t] + p);
+    float plo, phi;  // partner's value as seen by the lower / upper lane
+    if constexpr (M == 1) {
+      plo = phi = dpp_f<0xB1>(r[t]);
+    } else if constexpr (M == 2) {
+      plo = phi = dpp_f<0x4E>(r[t]);
+    } else if constexpr (M == 4) {
+      plo = dpp_f<0x104>(r[t]);
+      phi = dpp_f<0x114>(r[t]);
+    } else if constexpr (M == 8) {
+      plo = phi = dpp_f<0x128>(r[t]);
+    } else {
+      plo = phi = __shfl_xor(r[t], M);
+    }
+    r[t] = hi ? (phi - r[t]) : (r[t] + plo);
   }
 }
 
@@ -45,8 +75,9 @@ __device__ __forceinline__ void lane_butterfly(float (&r)[TP], int m, int lane) 
 // ascending step order.
 template <int G, int TP>
 __device__ __forceinline__ void wht_group(float (&r)[TP], int lane) {
-#pragma unroll
-  for (int m = 1; m < 8; m <<= 1) lane_butterfly(r, m, lane);
+  lane_butterfly<TP, 1>(r, lane);
+  lane_butterfly<TP, 2>(r, lane);
+  lane_butterfly<TP, 4>(r, lane);
 #pragma unroll
   for (int st = 1; st < TP; st <<= 1) {
 #pragma unroll
@@ -58,8 +89,9 @@ __device__ __forceinline__ void wht_group(float (&r)[TP], int lane) {
       }
     }
   }
-#pragma unroll
-  for (int m = 8; m < G; m <<= 1) lane_butterfly(r, m, lane);
+  if constexpr (G > 8) lane_butterfly<TP, 8>(r, lane);
+  if constexpr (G > 16) lane_butterfly<TP, 16>(r, lane);
+  if constexpr (G > 32) lane_butterfly<TP, 32>(r, lane);
 }
 
 // Sum of squares over the first `nblocks` blocks of 8 coordinates, in the
@@ -246,16 +278,16 @@ __global__ __launch_bounds__(256) void k_encode(SpaceDev s, const float *__restr
       if constexpr (NIB) {
         // word (h*TP + t) holds coordinates 8*(h*TP+t) .. +7, nibble j.
         uint32_t w = (u & 0xFu) << (4 * j);
-        w |= __shfl_xor(w, 1);
-        w |= __shfl_xor(w, 2);
-        w |= __shfl_xor(w, 4);
+        w |= dpp_u<0xB1>(w);
+        w |= dpp_u<0x4E>(w);
+        w |= dpp_u<0x141>(w);  // the other quad of the 8 lanes
         if (active && (t & 7) == j)
           *reinterpret_cast<uint32_t *>(out + 4 * (h * TP + t)) = w;
       } else {
         // word 2*(h*TP+t) + (j>>2) holds coordinates 4*word .. +3.
         uint32_t w = u << (8 * (j & 3));
-        w |= __shfl_xor(w, 1);
-        w |= __shfl_xor(w, 2);
+        w |= dpp_u<0xB1>(w);
+        w |= dpp_u<0x4E>(w);
         if (active && (t & 3) == (j & 3))
           *reinterpret_cast<uint32_t *>(out + 4 * (2 * (h * TP + t) + (j >> 2))) = w;
       }
