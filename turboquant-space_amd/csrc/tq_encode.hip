@@ -250,6 +250,68 @@ __global__ __launch_bounds__(256) void k_encode(SpaceDev s, const float *__restr
     if (sigma < 1e-10f) sigma = 1e-10f;
     const float inv_sigma = 1.0f / sigma;
 
+    uint8_t *out = codes + (active ? vec : 0L) * (long)s.cs;
+    if constexpr (NIB && TP >= 8) {
+      // Lloyd-Max index and residual (:361-367). At most 8 centroids: the
+      // count #{k : v > b_k} is a three-level search over the boundaries
+      // padded to 7 with +inf (the padding never compares true), each
+      // level's threshold picked by selects from registers.
+      float b[7];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) b[k] = k < s.L - 1 ? l_bnd[k] : __builtin_inff();
+      // nw[k] gathers the units of t = 8k + m as nibble m.
+      uint32_t nw[TP / 8];
+#pragma unroll
+      for (int k = 0; k < TP / 8; ++k) nw[k] = 0u;
+#pragma unroll
+      for (int t = 0; t < TP; ++t) {
+        const float v = r[t] * inv_sigma;
+        const bool c1 = v > b[3];
+        const bool c2 = v > (c1 ? b[5] : b[1]);
+        const bool c3 = v > (c1 ? (c2 ? b[6] : b[4]) : (c2 ? b[2] : b[0]));
+        const uint32_t q = (c1 ? 4u : 0u) | (c2 ? 2u : 0u) | (c3 ? 1u : 0u);
+        nw[t >> 3] |= q << (4 * (t & 7) + 1);
+        r[t] = fmaf(-l_cent[q], sigma, r[t]);
+      }
+      const float gamma = sqrtf(chain_sumsq<G, TP>(r, G / 8 * TP, h, j, lane, grp0));
+#pragma unroll
+      for (int t = 0; t < TP; ++t) r[t] = r[t] * l_qjl[8 * (h * TP + t) + j];
+      wht_group<G, TP>(r, lane);
+#pragma unroll
+      for (int t = 0; t < TP; ++t)
+        nw[t >> 3] |= (r[t] * s.inv_sqrt_D >= 0.0f ? 1u : 0u) << (4 * (t & 7));
+      // Lane j holds nibble j of words h*TP + 8k + m (m = 0..7); an 8x8
+      // nibble transpose over the 8 lanes j (swapping lane bit 2 with nibble
+      // bit 2, then bit 1, then bit 0) leaves lane j with word h*TP + 8k + j.
+      const bool j4 = (j & 4) != 0, j2 = (j & 2) != 0, j1 = (j & 1) != 0;
+      const uint32_t selA = j4 ? 0x07060302u : 0x01000504u;  // halves
+      const uint32_t selB = j2 ? 0x07030501u : 0x02060004u;  // bytes
+      const uint32_t rotC = j1 ? 4u : 28u;                   // nibbles
+      const uint32_t keepC = j1 ? 0xF0F0F0F0u : 0x0F0F0F0Fu;
+#pragma unroll
+      for (int k = 0; k < TP / 8; ++k) {
+        uint32_t w = nw[k];
+        // Both reads run in every lane (a DPP read from a lane switched off
+        // by a branch returns 0); each lane then keeps its partner's.
+        const uint32_t pl = dpp_u<0x104>(w), ph = dpp_u<0x114>(w);
+        uint32_t p = j4 ? ph : pl;
+        w = __builtin_amdgcn_perm(w, p, selA);
+        p = dpp_u<0x4E>(w);
+        w = __builtin_amdgcn_perm(w, p, selB);
+        p = dpp_u<0xB1>(w);
+        p = __builtin_amdgcn_alignbit(p, p, rotC);
+        w = (w & keepC) | (p & ~keepC);
+        if (active) *reinterpret_cast<uint32_t *>(out + 4 * (h * TP + 8 * k + j)) = w;
+      }
+      if (active && lane == grp0) {
+        float *meta = reinterpret_cast<float *>(out + s.packed);
+        meta[0] = norm;
+        meta[1] = gamma;
+        meta[2] = sigma;
+      }
+      return;
+    }
+
     // Lloyd-Max index and residual (:361-367).
     uint32_t idxw[(TP + 3) / 4];
 #pragma unroll
@@ -269,7 +331,6 @@ __global__ __launch_bounds__(256) void k_encode(SpaceDev s, const float *__restr
     wht_group<G, TP>(r, lane);
 
     // Pack units (idx<<1)|sign (:375-391) and write the trailer (:393-395).
-    uint8_t *out = codes + (active ? vec : 0L) * (long)s.cs;
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
       uint32_t q = (idxw[t >> 2] >> (8 * (t & 3))) & 0xFFu;
