@@ -15,13 +15,15 @@
 //    four bytes of a code word at once as (w & 0xEE..) + (w & 0x0E..) (no
 //    carries between bytes), and one v_perm per byte splices it under the
 //    table's base.
-//  * QJL term as an exact integer dot product: s_q is rounded to a 15-bit
-//    fixed-point integer (scale 2^e per query) split into four signed 4-bit
-//    digits, and sum_{qjl bit set} S_i is four v_dot8 (i4 x i4 -> i32) per
-//    code word against the word masked to its sign bits; then
-//    dot = (2*T - sum_i S_i) * 2^-e. The rounding of s_q to 15 bits moves
-//    the distance by ~1e-7 of its scale (the QJL term carries a factor
-//    scale*gamma ~ 1e-2).
+//  * QJL term as an exact integer dot product: s_q is rounded to a 19-bit
+//    fixed-point integer (scale 2^e per query, |S| < 2^18) split into five
+//    signed 4-bit digits, and sum_{qjl bit set} S_i is five v_dot8
+//    (i4 x i4 -> i32) per code word against the word masked to its sign
+//    bits; then dot = (2*T - sum_i S_i) * 2^-e. The rounding error of each
+//    S_i is bounded by 2^-19 of max|s_q|, not of s_i, so it weighs most at
+//    small D (few, large coordinates): at D = 128 it moves distances by
+//    ~7e-8 of (||q||^2 + ||x||^2) on average (15 bits: 1.1e-6 mean,
+//    1.04e-5 max over 320k distances, past the 1e-5 bar).
 // The plain fp32 path and its 1e-5 bar are unchanged: the per-code epilogue
 // is the reference's fp32 expression.
 #include "tq_common.h"
@@ -29,9 +31,11 @@
 namespace tq {
 namespace {
 
-// Per query: [e, S_int, 0, 0 | D/2 words]; word 4*wi + j holds digit j of the
-// fixed-point s_q of coordinates 8*wi .. 8*wi+7 (nibble v = coordinate 8wi+v).
-__host__ __device__ inline long qlimb_stride(int D) { return 4 + D / 2; }
+// Per query: [e, S_int, 0, 0 | kQjlDigits planes of D/8 words]; word wi of
+// plane j holds digit j of the fixed-point s_q of coordinates 8*wi .. 8*wi+7
+// (nibble v = coordinate 8wi+v).
+constexpr int kQjlDigits = 5;
+__host__ __device__ inline long qlimb_stride(int D) { return 4 + kQjlDigits * (D / 8); }
 
 __global__ __launch_bounds__(256) void k_qjl_limbs(SpaceDev s, const float *__restrict__ prep,
                                                    long m, int32_t *__restrict__ ql) {
@@ -52,21 +56,22 @@ __global__ __launch_bounds__(256) void k_qjl_limbs(SpaceDev s, const float *__re
     mx = red_f[0];
     int ex = 0;
     if (mx > 0.0f) frexpf(mx, &ex);  // mx < 2^ex
-    const int e = mx > 0.0f ? 14 - ex : 0;  // |s| * 2^e < 2^14
+    // |s| * 2^e < 2^(4*digits - 2): balanced base-16 digits in [-8, 7] hold it
+    const int e = mx > 0.0f ? 4 * kQjlDigits - 2 - ex : 0;
     int32_t *out = ql + q * lstride;
     int ssum = 0;
     for (int wi = threadIdx.x; wi < s.D / 8; wi += blockDim.x) {
-      uint32_t L[4] = {0u, 0u, 0u, 0u};
+      uint32_t L[kQjlDigits] = {};
       for (int v = 0; v < 8; ++v) {
         int S = (int)rintf(ldexpf(sq[8 * wi + v], e));
         ssum += S;
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < kQjlDigits; ++j) {
           const int d = ((S + 8) & 15) - 8;
           L[j] |= (uint32_t)(d & 15) << (4 * v);
           S = (S - d) >> 4;
         }
       }
-      for (int j = 0; j < 4; ++j) out[4 + 4 * wi + j] = (int32_t)L[j];
+      for (int j = 0; j < kQjlDigits; ++j) out[4 + j * (s.D / 8) + wi] = (int32_t)L[j];
     }
     red_i[threadIdx.x] = ssum;
     __syncthreads();
@@ -123,7 +128,7 @@ struct LutShape {
   static_assert(V == 1 || V == 2 || V == 4 || V == 8, "lanes per code");
   static constexpr size_t lds_bytes = sizeof(float) * 64 * (D / 2) +  // pair tables
                                       2 * 2 * 4 * (size_t)SL * CODES +  // slice partials
-                                      sizeof(int32_t) * (D / 2);        // QJL digit words
+                                      sizeof(int32_t) * kQjlDigits * (D / 8);  // QJL digit words
   static_assert(lds_bytes <= 160 * 1024, "LDS of one CU");
 };
 
@@ -157,7 +162,7 @@ __global__ __launch_bounds__(LutShape<D>::THREADS) void k_lut_nib(SpaceDev s, co
   __shared__ float tab[(D / 2) * 64];
   __shared__ float part_m[2][SL][CODES];
   __shared__ int part_q[2][SL][CODES];
-  __shared__ __align__(16) int32_t ql_lds[D / 2];
+  __shared__ __align__(16) int32_t ql_lds[kQjlDigits * (D / 8)];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -186,23 +191,23 @@ __global__ __launch_bounds__(LutShape<D>::THREADS) void k_lut_nib(SpaceDev s, co
       tab[e] = fmaf(pq[2 * p + 1], cent[hi], pq[2 * p] * cent[lo]);
     }
     if constexpr (!SYM)
-      for (int e = tid; e < D / 2; e += THREADS) ql_lds[e] = L[4 + e];
+      for (int e = tid; e < kQjlDigits * (D / 8); e += THREADS) ql_lds[e] = L[4 + e];
     __syncthreads();
     const float qns = pq[2 * s.D];
     const float qn = pq[2 * s.D + 1];
     const int qe = SYM ? 0 : L[0];
     const int qsum = SYM ? 0 : L[1];
     float *orow = out + qi * n;
-    // This lane's digit words (4 words x 4 digits), in registers for the scan.
-    int32_t lw[SYM ? 1 : 4][4];
+    // This lane's digit words (4 words x 5 digits), in registers for the scan.
+    int32_t lw[SYM ? 1 : 4][kQjlDigits];
     if constexpr (!SYM) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int4 v = *reinterpret_cast<const int4 *>(&ql_lds[4 * (w0 + u)]);
-        lw[u][0] = v.x;
-        lw[u][1] = v.y;
-        lw[u][2] = v.z;
-        lw[u][3] = v.w;
+      for (int j = 0; j < kQjlDigits; ++j) {
+        const int4 v = *reinterpret_cast<const int4 *>(&ql_lds[j * (D / 8) + w0]);
+        lw[0][j] = v.x;
+        lw[1][j] = v.y;
+        lw[2][j] = v.z;
+        lw[3][j] = v.w;
       }
     }
 
@@ -248,7 +253,7 @@ __global__ __launch_bounds__(LutShape<D>::THREADS) void k_lut_nib(SpaceDev s, co
 #pragma unroll
       for (int i = 0; i < V; ++i) {
         float am0 = 0.0f, am1 = 0.0f;
-        int aq0 = 0, aq1 = 0, aq2 = 0, aq3 = 0;
+        int aq0 = 0, aq1 = 0, aq2 = 0, aq3 = 0, aq4 = 0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const uint4 q4 = ring[k][i];
@@ -270,10 +275,11 @@ __global__ __launch_bounds__(LutShape<D>::THREADS) void k_lut_nib(SpaceDev s, co
             aq1 = __builtin_amdgcn_sdot8(t, lw[u][1], aq1, false);
             aq2 = __builtin_amdgcn_sdot8(t, lw[u][2], aq2, false);
             aq3 = __builtin_amdgcn_sdot8(t, lw[u][3], aq3, false);
+            aq4 = __builtin_amdgcn_sdot8(t, lw[u][4], aq4, false);
           }
         }
         const float am = group_sum_f<V>(am0 + am1);
-        const int aq = SYM ? 0 : group_sum_i<V>(aq0 + 16 * aq1 + 256 * aq2 + 4096 * aq3);
+        const int aq = SYM ? 0 : group_sum_i<V>(aq0 + 16 * aq1 + 256 * aq2 + 4096 * aq3 + 65536 * aq4);
         if (chunk == i % V) {
           const int slot_i = grp * 64 + i * (64 / V) + lane / V;
           part_m[buf][sl][slot_i] = am;
