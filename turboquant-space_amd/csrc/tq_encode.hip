@@ -312,6 +312,67 @@ __global__ __launch_bounds__(256) void k_encode(SpaceDev s, const float *__restr
       return;
     }
 
+    if constexpr (!NIB && TP >= 4) {
+      // Lloyd-Max index and residual (:361-367) for byte codes (up to 255
+      // boundaries): binary search in LDS, QG coordinates side by side so
+      // each level issues QG independent lookups. bw[k] gathers the units of
+      // t = 4k + m as byte m; (idx << 1) keeps 8 bits as the reference's u8
+      // unit does (:388-389).
+      constexpr int QG = TP < 8 ? TP : 8;
+      uint32_t bw[TP / 4];
+#pragma unroll
+      for (int k = 0; k < TP / 4; ++k) bw[k] = 0u;
+#pragma unroll
+      for (int t0 = 0; t0 < TP; t0 += QG) {
+        float v[QG];
+        int idx[QG];
+#pragma unroll
+        for (int e = 0; e < QG; ++e) {
+          v[e] = r[t0 + e] * inv_sigma;
+          idx[e] = 0;
+        }
+        for (int k = s.mse_bits - 1; k >= 0; --k) {
+          const int step = 1 << k;
+#pragma unroll
+          for (int e = 0; e < QG; ++e) idx[e] += v[e] > l_bnd[idx[e] + step - 1] ? step : 0;
+        }
+#pragma unroll
+        for (int e = 0; e < QG; ++e) {
+          const int t = t0 + e;
+          bw[t >> 2] |= ((uint32_t)(idx[e] << 1) & 0xFFu) << (8 * (t & 3));
+          r[t] = fmaf(-l_cent[idx[e]], sigma, r[t]);
+        }
+      }
+      const float gamma = sqrtf(chain_sumsq<G, TP>(r, G / 8 * TP, h, j, lane, grp0));
+#pragma unroll
+      for (int t = 0; t < TP; ++t) r[t] = r[t] * l_qjl[8 * (h * TP + t) + j];
+      wht_group<G, TP>(r, lane);
+#pragma unroll
+      for (int t = 0; t < TP; ++t)
+        bw[t >> 2] |= (r[t] * s.inv_sqrt_D >= 0.0f ? 1u : 0u) << (8 * (t & 3));
+      // Lane j holds byte j & 3 of words 2*(h*TP + 4k + m) + (j >> 2)
+      // (m = 0..3); a 4x4 byte transpose over the lanes of a quad (lane bit 1
+      // with byte bit 1, then bit 0 with bit 0) leaves lane j with word
+      // 2*(h*TP + 4k + (j & 3)) + (j >> 2).
+      const uint32_t selA = (j & 2) ? 0x07060302u : 0x01000504u;  // halves
+      const uint32_t selB = (j & 1) ? 0x07030501u : 0x02060004u;  // bytes
+#pragma unroll
+      for (int k = 0; k < TP / 4; ++k) {
+        uint32_t w = bw[k];
+        w = __builtin_amdgcn_perm(w, dpp_u<0x4E>(w), selA);
+        w = __builtin_amdgcn_perm(w, dpp_u<0xB1>(w), selB);
+        if (active)
+          *reinterpret_cast<uint32_t *>(out + 4 * (2 * (h * TP + 4 * k + (j & 3)) + (j >> 2))) = w;
+      }
+      if (active && lane == grp0) {
+        float *meta = reinterpret_cast<float *>(out + s.packed);
+        meta[0] = norm;
+        meta[1] = gamma;
+        meta[2] = sigma;
+      }
+      return;
+    }
+
     // Lloyd-Max index and residual (:361-367).
     uint32_t idxw[(TP + 3) / 4];
 #pragma unroll
