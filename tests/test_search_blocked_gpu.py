@@ -1,0 +1,138 @@
+"""MFMA search over collections of at least 2,000,000 codes, which plans run
+blocked (MFMA distance blocks over chunks of codes, row top-k per chunk,
+folded into one list per query) instead of through the fused persistent
+kernel.
+
+Bars: the plan's top-k equals the (distance, id)-ordered top-k of the full
+MFMA distance matrix, distances bit for bit, with one chunk and with several
+(m = 5, 37, 100 over 2.3M codes: 1, 2 and 4 chunks); a plan holding a blocked
+and a fused batch gives each job what it gets alone; graph replays equal eager
+runs; the server over such a collection returns what a one-shot plan returns.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N_BIG = 2_300_000
+K = 10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def big(gpu):
+    import torch
+    capi = gpu
+    sp = capi.Space(128, 4)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    codes = torch.empty((N_BIG, sp.cs), dtype=torch.uint8, device="cuda")
+    for s0 in range(0, N_BIG, 500_000):
+        rows = min(500_000, N_BIG - s0)
+        x = torch.randn((rows, 128), generator=g, device="cuda")
+        sp.encode(x, rows, codes[s0:s0 + rows])
+    # a few exact duplicates, so equal distances occur and ties go by id
+    codes[N_BIG - 1000:] = codes[:1000]
+    torch.cuda.synchronize()
+    yield sp, codes, g
+    sp.close()
+
+
+def matrix_topk(torch, capi, sp, Qd, m, codes, n, k):
+    """(distance, id) top-k of the MFMA matrix, on the device: a stable sort
+    keeps equal distances in id order."""
+    D = torch.empty((m, n), dtype=torch.float32, device="cuda")
+    sp.m_to_n(Qd, m, codes, n, D, mode=capi.TQ_MODE_MFMA)
+    vals, idx = torch.sort(D, dim=1, stable=True)
+    out = vals[:, :k].cpu().numpy(), idx[:, :k].cpu().numpy()
+    del D, vals, idx
+    torch.cuda.empty_cache()
+    return out
+
+
+def plan_topk(torch, capi, jobs, k, graph_replays=0):
+    outs = []
+    spec = []
+    for sp, Qd, m, codes, n in jobs:
+        od = torch.empty((m, k), dtype=torch.float32, device="cuda")
+        oi = torch.empty((m, k), dtype=torch.int64, device="cuda")
+        outs.append((od, oi))
+        spec.append((sp, Qd, m, codes, n, od, oi))
+    plan = capi.SearchPlan(spec, k)
+    plan.run()
+    torch.cuda.synchronize()
+    res = [(od.cpu().numpy(), oi.cpu().numpy()) for od, oi in outs]
+    for _ in range(graph_replays):
+        for od, oi in outs:
+            od.fill_(-7.0)
+            oi.fill_(-7)
+        torch.cuda.synchronize()
+        plan.run_graph()
+        torch.cuda.synchronize()
+        again = [(od.cpu().numpy(), oi.cpu().numpy()) for od, oi in outs]
+        for (a_d, a_i), (b_d, b_i) in zip(res, again):
+            np.testing.assert_array_equal(a_i, b_i)
+            np.testing.assert_array_equal(a_d, b_d)
+    plan.close()
+    return res
+
+
+@pytest.mark.parametrize("m", [5, 37, 100])
+def test_blocked_equals_topk_of_matrix(gpu, big, m):
+    import torch
+    capi = gpu
+    sp, codes, g = big
+    say(f"[gpu] blocked search m={m} n={N_BIG}")
+    Qd = torch.randn((m, 128), generator=g, device="cuda")
+    want_d, want_i = matrix_topk(torch, capi, sp, Qd, m, codes, N_BIG, K)
+    (got_d, got_i), = plan_topk(torch, capi, [(sp, Qd, m, codes, N_BIG)], K,
+                                graph_replays=2 if m == 37 else 0)
+    np.testing.assert_array_equal(got_i, want_i)
+    np.testing.assert_array_equal(got_d, want_d)
+
+
+def test_mixed_blocked_and_fused(gpu, big):
+    import torch
+    capi = gpu
+    sp, codes, g = big
+    say("[gpu] blocked + fused batches in one plan")
+    sp2 = capi.Space(96, 4)
+    n2 = 300_000
+    x2 = torch.randn((n2, 96), generator=g, device="cuda")
+    codes2 = torch.empty((n2, sp2.cs), dtype=torch.uint8, device="cuda")
+    sp2.encode(x2, n2, codes2)
+    Q1 = torch.randn((20, 128), generator=g, device="cuda")
+    Q2 = torch.randn((33, 96), generator=g, device="cuda")
+    Q3 = torch.randn((7, 128), generator=g, device="cuda")
+    jobs = [(sp, Q1, 20, codes, N_BIG), (sp2, Q2, 33, codes2, n2), (sp, Q3, 7, codes, N_BIG)]
+    mixed = plan_topk(torch, capi, jobs, K)
+    for job, (md, mi) in zip(jobs, mixed):
+        (ad, ai), = plan_topk(torch, capi, [job], K)
+        np.testing.assert_array_equal(mi, ai)
+        np.testing.assert_array_equal(md, ad)
+    sp2.close()
+
+
+def test_server_over_blocked_collection(gpu, big):
+    import torch
+    capi = gpu
+    sp, codes, g = big
+    say("[gpu] server over a blocked collection")
+    srv = capi.Server(mode=capi.TQ_MODE_MFMA, k=K, hbm_budget=8 << 30, slot_bytes=1 << 30,
+                      max_inflight=2, max_batch_queries=4096)
+    cid = srv.add_collection(sp, N_BIG)
+    srv.load_codes(cid, codes, N_BIG)
+    Qd = torch.randn((37, 128), generator=g, device="cuda")
+    Q = Qd.cpu().numpy()
+    od = np.empty((37, K), np.float32)
+    oi = np.empty((37, K), np.int64)
+    srv.wait(srv.search(cid, Q, od, oi))
+    srv.close()
+    (pd, pi), = plan_topk(torch, capi, [(sp, Qd, 37, codes, N_BIG)], K)
+    np.testing.assert_array_equal(oi, pi)
+    np.testing.assert_array_equal(od, pd)

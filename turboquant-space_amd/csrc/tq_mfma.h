@@ -13,4 +13,8 @@ hipError_t launch_scan_mfma(const SpaceDev &s, const float *prep, long m,
 // each 8-coordinate group in the nibble decoder's order).
 hipError_t launch_split_queries(const SpaceDev &s, const float *prep, long m,
                                 _Float16 *planes, hipStream_t st);
+// The MFMA distances [m][n] of queries already split into `planes` (the
+// layout launch_split_queries writes, m <= 65535 * 64), no allocation.
+hipError_t launch_mfma_planes(const SpaceDev &s, const _Float16 *planes, const float *prep, long m,
+                              const uint8_t *codes, long n, float *out, hipStream_t st);
 }  // namespace tq

@@ -338,6 +338,20 @@ hipError_t launch_split_queries(const SpaceDev &s, const float *prep, long m, _F
   return hipGetLastError();
 }
 
+hipError_t launch_mfma_planes(const SpaceDev &s, const _Float16 *planes, const float *prep, long m,
+                              const uint8_t *codes, long n, float *out, hipStream_t st) {
+  if (s.D < kKC || m > 65535L * kTQ) return hipErrorNotSupported;
+  if (m <= 0 || n <= 0) return hipSuccess;
+  dim3 grid((unsigned)((n + kTN - 1) / kTN), (unsigned)((m + kTQ - 1) / kTQ));
+  if (s.nib && s.mfma_lo)
+    hipLaunchKernelGGL((k_mfma<true, true>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+  else if (s.nib)
+    hipLaunchKernelGGL((k_mfma<true, false>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+  else
+    hipLaunchKernelGGL((k_mfma<false, true>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_scan_mfma(const SpaceDev &s, const float *prep, long m, const uint8_t *codes,
                             long n, float *out, hipStream_t st) {
   if (s.D < kKC) return hipErrorNotSupported;
@@ -360,14 +374,7 @@ hipError_t launch_scan_mfma(const SpaceDev &s, const float *prep, long m, const 
                        planes, s.nib);
     e = hipGetLastError();
     if (e != hipSuccess) break;
-    dim3 grid((unsigned)((n + kTN - 1) / kTN), (unsigned)((mc + kTQ - 1) / kTQ));
-    if (s.nib && s.mfma_lo)
-      hipLaunchKernelGGL((k_mfma<true, true>), grid, dim3(256), 0, st, s, planes, pc, mc, codes, n, oc);
-    else if (s.nib)
-      hipLaunchKernelGGL((k_mfma<true, false>), grid, dim3(256), 0, st, s, planes, pc, mc, codes, n, oc);
-    else
-      hipLaunchKernelGGL((k_mfma<false, true>), grid, dim3(256), 0, st, s, planes, pc, mc, codes, n, oc);
-    e = hipGetLastError();
+    e = launch_mfma_planes(s, planes, pc, mc, codes, n, oc, st);
   }
   hipError_t f = hipFreeAsync(planes, st);
   return e != hipSuccess ? e : f;

@@ -47,5 +47,15 @@ hipError_t launch_search(const SearchJobDev *d_jobs, const SearchItem *d_items,
 long topk_rows_segments(long m, long n);
 hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out_d,
                             int64_t *out_i, float *seg_d, uint32_t *seg_i, hipStream_t st);
+// The same over one chunk of columns, kept as partial lists: the
+// topk_rows_segments(m, n) sorted lists of each row of the [m][n] block go to
+// lists seg0 .. of that row's `stride` lists (entries of kSearchMaxK), with
+// ids = column + id0. launch_topk_rows_fold then merges lists [l0, l1) of
+// every row either into list 0 (out_d == nullptr) or into the final k per
+// row at out_d / out_i (ids ~0 -> -1).
+hipError_t launch_topk_rows_chunk(const float *dist, long m, long n, long stride, long seg0,
+                                  uint32_t id0, float *seg_d, uint32_t *seg_i, hipStream_t st);
+hipError_t launch_topk_rows_fold(long m, long stride, long l0, long l1, int k, float *seg_d,
+                                 uint32_t *seg_i, float *out_d, int64_t *out_i, hipStream_t st);
 
 }  // namespace tq

@@ -438,38 +438,73 @@ __global__ __launch_bounds__(256) void k_topk_merge(const SearchJobDev *__restri
   wave_pop_topk<K>(td, ti, k_out, lane, job.out_d + q * k_out, job.out_i + q * k_out);
 }
 
-// Row-wise top-k of a dense distance matrix, stage 1: workgroup (seg, row)
-// scans columns [seg*span, (seg+1)*span) of the row, each lane keeping the
-// best K of a strided subset; the 4 wavefronts' 256 lists are merged through
-// LDS by wavefront 0 into one sorted list per (row, seg).
+// Row-wise top-k of a dense distance matrix, stage 1: wavefront w of
+// workgroup (x, row) scans the columns of segment seg = 4x + w of the row
+// ([seg*span, (seg+1)*span)), each lane keeping the best K of a strided
+// subset; the wavefront then pops the best K of its 64 lists by wavefront
+// minima and stores them as list seg0 + seg of the row's `stride` lists,
+// with ids = column + id0. No LDS and no barrier: a wavefront's segment is
+// all it touches.
 template <int K>
 __global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dist, long n,
-                                                   long span, float *__restrict__ seg_d,
+                                                   long span, long nseg, long stride, long seg0,
+                                                   uint32_t id0, float *__restrict__ seg_d,
                                                    uint32_t *__restrict__ seg_i) {
-  __shared__ float s_d[256 * K];
-  __shared__ uint32_t s_i[256 * K];
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const long seg = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (seg >= nseg) return;  // whole wavefront
   const long row = blockIdx.y;
-  const long c0 = (long)blockIdx.x * span;
+  const long c0 = seg * span;
   const long c1 = c0 + span < n ? c0 + span : n;
   const float *p = dist + row * n;
   float td[K];
   uint32_t ti[K];
   topk_clear<K>(td, ti);
-  for (long j = c0 + tid; j < c1; j += 256) {
-    const float d = p[j];
-    if (lex_less(d, (uint32_t)j, td[K - 1], ti[K - 1])) topk_insert<K>(td, ti, d, (uint32_t)j);
-  }
+  // Every lane's K-th entry has K entries of the row at or below it, so an
+  // entry above the wavefront's smallest K-th entry (thr) cannot make the
+  // top K: most entries are rejected by one compare once the lists fill.
+  // The loop runs the same trip count in every lane (thr is refreshed with
+  // cross-lane reads); columns past c1 are never taken. U columns per lane
+  // are loaded before any is used.
+  constexpr int U = 8;
+  float thr_d = __builtin_inff();
+  uint32_t thr_i = 0xFFFFFFFFu;
+  const long iters = c1 > c0 ? (c1 - c0 + 63) / 64 : 0;
+  for (long it = 0; it < iters; it += U) {
+    float v[U];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    s_d[tid * K + k] = td[k];
-    s_i[tid * K + k] = ti[k];
+    for (int u = 0; u < U; ++u) {
+      const long j = c0 + (it + u) * 64 + lane;
+      v[u] = j < c1 ? p[j] : __builtin_inff();
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long j = c0 + (it + u) * 64 + lane;
+      const bool ok = j < c1;
+      const float d = v[u];
+      const uint32_t id = (uint32_t)j + id0;
+      const bool take =
+          ok && lex_less(d, id, thr_d, thr_i) && lex_less(d, id, td[K - 1], ti[K - 1]);
+      if (__any(take)) {
+        if (take) topk_insert<K>(td, ti, d, id);
+      }
+    }
+    float bd = td[K - 1];
+    uint32_t bi = ti[K - 1];
+    for (int o = 32; o > 0; o >>= 1) {
+      const float od = __shfl_xor(bd, o);
+      const uint32_t oi = __shfl_xor(bi, o);
+      if (lex_less(od, oi, bd, bi)) {
+        bd = od;
+        bi = oi;
+      }
+    }
+    if (lex_less(bd, bi, thr_d, thr_i)) {
+      thr_d = bd;
+      thr_i = bi;
+    }
   }
-  __syncthreads();
-  if (tid >= 64) return;
-  topk_clear<K>(td, ti);
-  for (int l = tid; l < 256; l += 64) take_sorted<K>(td, ti, s_d + l * K, s_i + l * K);
-  const long base = (row * gridDim.x + blockIdx.x) * K;
+  const long base = (row * stride + seg0 + seg) * K;
   // K pops write the segment's sorted list (ids as u32, ~0 = empty).
   for (int k = 0; k < K; ++k) {
     float bd = td[0];
@@ -482,7 +517,7 @@ __global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dis
         bi = oi;
       }
     }
-    if (tid == 0) {
+    if (lane == 0) {
       seg_d[base + k] = bd;
       seg_i[base + k] = bi;
     }
@@ -516,6 +551,58 @@ __global__ __launch_bounds__(256) void k_topk_rows_merge(long m, int nseg, int k
     take_sorted<K>(td, ti, seg_d + base, seg_i + base);
   }
   wave_pop_topk<K>(td, ti, k_out, lane, out_d + row * k_out, out_i + row * k_out);
+}
+
+// Folds lists [l0, l1) of each row (`stride` lists per row) into one: one
+// wavefront per row takes them into lane lists, then pops either the row's
+// best k_out into out_d / out_i (when out_d is set) or its best K back into
+// list 0 as a sorted list. Only lane 0 reads list 0, before any pop writes it.
+template <int K>
+__global__ __launch_bounds__(256) void k_topk_rows_fold(long m, long stride, long l0, long l1,
+                                                        int k_out, float *__restrict__ seg_d,
+                                                        uint32_t *__restrict__ seg_i,
+                                                        float *__restrict__ out_d,
+                                                        int64_t *__restrict__ out_i) {
+  const int lane = threadIdx.x & 63;
+  const long row = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= m) return;
+  float td[K];
+  uint32_t ti[K];
+  topk_clear<K>(td, ti);
+  for (long l = l0 + lane; l < l1; l += 64) {
+    const long base = (row * stride + l) * K;
+    take_sorted<K>(td, ti, seg_d + base, seg_i + base);
+  }
+  if (out_d) {
+    wave_pop_topk<K>(td, ti, k_out, lane, out_d + row * k_out, out_i + row * k_out);
+    return;
+  }
+  const long base = row * stride * K;
+  for (int k = 0; k < K; ++k) {
+    float bd = td[0];
+    uint32_t bi = ti[0];
+    for (int o = 32; o > 0; o >>= 1) {
+      const float od = __shfl_xor(bd, o);
+      const uint32_t oi = __shfl_xor(bi, o);
+      if (lex_less(od, oi, bd, bi)) {
+        bd = od;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      seg_d[base + k] = bd;
+      seg_i[base + k] = bi;
+    }
+    if (td[0] == bd && ti[0] == bi) {
+#pragma unroll
+      for (int t = 0; t + 1 < K; ++t) {
+        td[t] = td[t + 1];
+        ti[t] = ti[t + 1];
+      }
+      td[K - 1] = __builtin_inff();
+      ti[K - 1] = 0xFFFFFFFFu;
+    }
+  }
 }
 
 }  // namespace
@@ -555,10 +642,10 @@ hipError_t launch_search(const SearchJobDev *d_jobs, const SearchItem *d_items,
 }
 
 long topk_rows_segments(long m, long n) {
-  // About 2048 workgroups over the matrix, each segment at least 4096 columns
-  // (so m * segments < 2048 + m).
-  long seg = (2048 + m - 1) / m;
-  const long max_seg = (n + 4095) / 4096;
+  // About 8192 wavefronts over the matrix, each segment at least 2048
+  // columns (so m * segments < 8192 + m).
+  long seg = (8192 + m - 1) / m;
+  const long max_seg = (n + 2047) / 2048;
   if (seg > max_seg) seg = max_seg;
   if (seg < 1) seg = 1;
   if (seg > 65535) seg = 65535;
@@ -571,12 +658,32 @@ hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out
   if (k < 1 || k > kSearchMaxK || m > 65535 || n >= 0xFFFFFFFFL) return hipErrorInvalidValue;
   const long nseg = topk_rows_segments(m, n);
   const long span = (n + nseg - 1) / nseg;
-  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3((unsigned)nseg, (unsigned)m), dim3(256), 0, st,
-                     dist, n, span, seg_d, seg_i);
+  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3((unsigned)((nseg + 3) / 4), (unsigned)m),
+                     dim3(256), 0, st, dist, n, span, nseg, nseg, 0L, 0u, seg_d, seg_i);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_topk_rows_merge<kSearchMaxK>, dim3((unsigned)((m + 3) / 4)), dim3(256), 0,
                      st, m, (int)nseg, k, seg_d, seg_i, out_d, out_i);
+  return hipGetLastError();
+}
+
+hipError_t launch_topk_rows_chunk(const float *dist, long m, long n, long stride, long seg0,
+                                  uint32_t id0, float *seg_d, uint32_t *seg_i, hipStream_t st) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  const long nseg = topk_rows_segments(m, n);
+  if (m > 65535 || n >= 0xFFFFFFFFL || seg0 + nseg > stride) return hipErrorInvalidValue;
+  const long span = (n + nseg - 1) / nseg;
+  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3((unsigned)((nseg + 3) / 4), (unsigned)m),
+                     dim3(256), 0, st, dist, n, span, nseg, stride, seg0, id0, seg_d, seg_i);
+  return hipGetLastError();
+}
+
+hipError_t launch_topk_rows_fold(long m, long stride, long l0, long l1, int k, float *seg_d,
+                                 uint32_t *seg_i, float *out_d, int64_t *out_i, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  if (k < 1 || k > kSearchMaxK || l0 < 0 || l1 > stride || l0 >= l1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_topk_rows_fold<kSearchMaxK>, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, st,
+                     m, stride, l0, l1, k, seg_d, seg_i, out_d, out_i);
   return hipGetLastError();
 }
 

@@ -228,9 +228,17 @@ size_t batch_bound(tq_server_s *s, const std::vector<Request *> &b) {
     qtiles += (r->rows + 31) / 32;
   }
   if (any) {
-    if (s->cfg.mode == TQ_MODE_MFMA)
+    if (s->cfg.mode == TQ_MODE_MFMA) {
       bytes += (3 * (size_t)s->cus * 8 + qtiles + 16) * 4 * 64 * TQ_SEARCH_MAX_K * 8;
-    else
+      long rows = 0;
+      bool blocked = false;
+      for (Request *r : b)
+        if (r->kind == kSearch) {
+          rows += (long)r->rows;
+          blocked = blocked || tq::plan_blocked((long)r->n_at, 1);
+        }
+      if (blocked) bytes += tq::plan_blocked_ws_bound(rows);
+    } else
       bytes += (512ull << 20) + (8ull << 20);
   }
   return bytes;
