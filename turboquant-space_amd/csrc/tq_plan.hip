@@ -351,8 +351,9 @@ int layout(tq_search_plan_s *p) {
     } else if (b.m > 0) {
       const long rows = fp32_block_rows(b);
       dist_elems = std::max(dist_elems, (size_t)rows * b.n);
-      // Any block of r <= rows rows uses r * topk_rows_segments(r, n) < 8192 + rows lists.
-      seg_elems = std::max(seg_elems, (size_t)(8192 + rows) * tq::kSearchMaxK);
+      // A block of r <= rows rows uses r * topk_rows_segments(r, n) lists.
+      seg_elems = std::max(seg_elems,
+                           (size_t)rows * tq::topk_rows_segments(rows, b.n) * tq::kSearchMaxK);
       scan_ws = std::max(scan_ws, tq::scan_fp32_ws_bytes(b.space->dev, rows));
     }
   }
@@ -636,10 +637,11 @@ bool plan_blocked(long n, long rows) {
 }
 
 size_t plan_blocked_ws_bound(long rows) {
-  // The distance block, and (1 + segments) lists per query: segments <=
-  // ceil(8192 / rows) <= 8192 / rows + 1.
-  return align_up(kMfmaBlockBytes) +
-         2 * align_up((size_t)(2 * rows + 8192) * kSearchMaxK * sizeof(float)) + 4096;
+  // The distance block, and (1 + segments) lists per query over a chunk of
+  // nc <= max(4096, kMfmaBlockBytes / (4 * rows)) codes, segments <=
+  // ceil(nc / 2048): rows * lists <= 4 * rows + kMfmaBlockBytes / 8192.
+  const size_t lists = 4 * (size_t)rows + kMfmaBlockBytes / 8192;
+  return align_up(kMfmaBlockBytes) + 2 * align_up(lists * kSearchMaxK * sizeof(float)) + 4096;
 }
 
 }  // namespace tq

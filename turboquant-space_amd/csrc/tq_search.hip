@@ -60,6 +60,41 @@ __device__ __forceinline__ void topk_insert(float (&td)[K], uint32_t (&ti)[K], f
   }
 }
 
+// Lexicographic minimum of (d, i) with the lane DPP control CTRL selects.
+template <int CTRL>
+__device__ __forceinline__ void lex_min_dpp(float &d, uint32_t &i) {
+  const float od = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(d), CTRL, 0xF, 0xF, true));
+  const uint32_t oi = (uint32_t)__builtin_amdgcn_mov_dpp((int)i, CTRL, 0xF, 0xF, true);
+  if (lex_less(od, oi, d, i)) {
+    d = od;
+    i = oi;
+  }
+}
+
+// Wavefront-wide lexicographic minimum of (d, i), returned to every lane:
+// DPP within each row of 16 lanes (quad_perm lane^1, lane^2, row_half_mirror
+// 7-i, row_mirror 15-i), then the four rows' results through readlane. All
+// lanes must be active. No LDS round trips (a shuffle tree costs six).
+__device__ __forceinline__ void wave_lex_min(float &d, uint32_t &i) {
+  lex_min_dpp<0xB1>(d, i);
+  lex_min_dpp<0x4E>(d, i);
+  lex_min_dpp<0x141>(d, i);
+  lex_min_dpp<0x140>(d, i);
+  float bd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), 0));
+  uint32_t bi = (uint32_t)__builtin_amdgcn_readlane((int)i, 0);
+#pragma unroll
+  for (int r = 1; r < 4; ++r) {
+    const float rd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), 16 * r));
+    const uint32_t ri = (uint32_t)__builtin_amdgcn_readlane((int)i, 16 * r);
+    if (lex_less(rd, ri, bd, bi)) {
+      bd = rd;
+      bi = ri;
+    }
+  }
+  d = bd;
+  i = bi;
+}
+
 template <int K>
 __device__ __forceinline__ void topk_clear(float (&td)[K], uint32_t (&ti)[K]) {
 #pragma unroll
@@ -363,14 +398,7 @@ __device__ void wave_pop_topk(float (&td)[K], uint32_t (&ti)[K], int k_out, int 
   for (int k = 0; k < k_out; ++k) {
     float bd = td[0];
     uint32_t bi = ti[0];
-    for (int o = 32; o > 0; o >>= 1) {
-      const float od = __shfl_xor(bd, o);
-      const uint32_t oi = __shfl_xor(bi, o);
-      if (lex_less(od, oi, bd, bi)) {
-        bd = od;
-        bi = oi;
-      }
-    }
+    wave_lex_min(bd, bi);
     if (lane == 0) {
       out_d[k] = bd;
       out_i[k] = bi == 0xFFFFFFFFu ? -1 : (int64_t)bi;
@@ -438,169 +466,278 @@ __global__ __launch_bounds__(256) void k_topk_merge(const SearchJobDev *__restri
   wave_pop_topk<K>(td, ti, k_out, lane, job.out_d + q * k_out, job.out_i + q * k_out);
 }
 
-// Row-wise top-k of a dense distance matrix, stage 1: wavefront w of
-// workgroup (x, row) scans the columns of segment seg = 4x + w of the row
-// ([seg*span, (seg+1)*span)), each lane keeping the best K of a strided
-// subset; the wavefront then pops the best K of its 64 lists by wavefront
-// minima and stores them as list seg0 + seg of the row's `stride` lists,
-// with ids = column + id0. No LDS and no barrier: a wavefront's segment is
-// all it touches.
-template <int K>
-__global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dist, long n,
-                                                   long span, long nseg, long stride, long seg0,
-                                                   uint32_t id0, float *__restrict__ seg_d,
-                                                   uint32_t *__restrict__ seg_i) {
-  const int lane = threadIdx.x & 63;
-  const long seg = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (seg >= nseg) return;  // whole wavefront
-  const long row = blockIdx.y;
-  const long c0 = seg * span;
-  const long c1 = c0 + span < n ? c0 + span : n;
-  const float *p = dist + row * n;
-  float td[K];
-  uint32_t ti[K];
-  topk_clear<K>(td, ti);
-  // Every lane's K-th entry has K entries of the row at or below it, so an
-  // entry above the wavefront's smallest K-th entry (thr) cannot make the
-  // top K: most entries are rejected by one compare once the lists fill.
-  // The loop runs the same trip count in every lane (thr is refreshed with
-  // cross-lane reads); columns past c1 are never taken. U columns per lane
-  // are loaded before any is used.
-  constexpr int U = 8;
-  float thr_d = __builtin_inff();
-  uint32_t thr_i = 0xFFFFFFFFu;
-  const long iters = c1 > c0 ? (c1 - c0 + 63) / 64 : 0;
-  for (long it = 0; it < iters; it += U) {
-    float v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const long j = c0 + (it + u) * 64 + lane;
-      v[u] = j < c1 ? p[j] : __builtin_inff();
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const long j = c0 + (it + u) * 64 + lane;
-      const bool ok = j < c1;
-      const float d = v[u];
-      const uint32_t id = (uint32_t)j + id0;
-      const bool take =
-          ok && lex_less(d, id, thr_d, thr_i) && lex_less(d, id, td[K - 1], ti[K - 1]);
-      if (__any(take)) {
-        if (take) topk_insert<K>(td, ti, d, id);
-      }
-    }
-    float bd = td[K - 1];
-    uint32_t bi = ti[K - 1];
-    for (int o = 32; o > 0; o >>= 1) {
-      const float od = __shfl_xor(bd, o);
-      const uint32_t oi = __shfl_xor(bi, o);
-      if (lex_less(od, oi, bd, bi)) {
-        bd = od;
-        bi = oi;
-      }
-    }
-    if (lex_less(bd, bi, thr_d, thr_i)) {
-      thr_d = bd;
-      thr_i = bi;
-    }
-  }
-  const long base = (row * stride + seg0 + seg) * K;
-  // K pops write the segment's sorted list (ids as u32, ~0 = empty).
+// K pops of the wavefront's best entries from lane-held sorted lists of L
+// entries; lane 0 stores them at (out_d, out_i) as a sorted list.
+template <int L, int K>
+__device__ __forceinline__ void wave_pop_list(float (&td)[L], uint32_t (&ti)[L], float *out_d,
+                                              uint32_t *out_i, int lane) {
   for (int k = 0; k < K; ++k) {
     float bd = td[0];
     uint32_t bi = ti[0];
-    for (int o = 32; o > 0; o >>= 1) {
-      const float od = __shfl_xor(bd, o);
-      const uint32_t oi = __shfl_xor(bi, o);
-      if (lex_less(od, oi, bd, bi)) {
-        bd = od;
-        bi = oi;
-      }
-    }
+    wave_lex_min(bd, bi);
     if (lane == 0) {
-      seg_d[base + k] = bd;
-      seg_i[base + k] = bi;
+      out_d[k] = bd;
+      out_i[k] = bi;
     }
-    if (td[0] == bd && ti[0] == bi) {
+    if (td[0] == bd && ti[0] == bi) {  // ids are unique; empty entries popping together is harmless
 #pragma unroll
-      for (int t = 0; t + 1 < K; ++t) {
+      for (int t = 0; t + 1 < L; ++t) {
         td[t] = td[t + 1];
         ti[t] = ti[t + 1];
       }
-      td[K - 1] = __builtin_inff();
-      ti[K - 1] = 0xFFFFFFFFu;
+      td[L - 1] = __builtin_inff();
+      ti[L - 1] = 0xFFFFFFFFu;
     }
   }
 }
 
-// Stage 2: one wavefront per row merges the row's segment lists.
+// Row-wise top-k of a dense distance matrix, stage 1: wavefront w of
+// workgroup (x, row) owns segments seg = 4x + w (+ 4 * gridDim.x ...) of
+// kTopkSeg columns, read as sub-blocks of kTopkSub columns (32 per lane, in
+// registers), and writes each segment's best K as list seg0 + seg of the
+// row's `stride` lists (ids = column + id0).
+//   1. Every lane's minimum over the first sub-block; thr = the K-th smallest
+//      of the 64 minima. They are K distinct entries of the segment, so the
+//      segment's K best all lie at or below thr.
+//   2. The entries at or below thr (on typical rows a few dozen per
+//      segment) are packed into a per-wavefront LDS list by ballot and lane
+//      prefix counts, sub-block by sub-block.
+//   3. Lanes take them four apiece, sort their four, and K wavefront-minimum
+//      pops give the sorted list. If more than kTopkCap entries lie at or
+//      below thr, the segment is read again and each lane sorts its entries
+//      at or below thr into a list of K (exact either way).
+// Loads are unconditional (column index clamped) so a sub-block's loads are
+// all in flight together. Only the wavefront's own lanes touch its LDS list:
+// no barrier.
+constexpr int kTopkSub = 2048;
+constexpr int kTopkSeg = 4 * kTopkSub;
+constexpr int kTopkCap = 256;
 template <int K>
-__global__ __launch_bounds__(256) void k_topk_rows_merge(long m, int nseg, int k_out,
-                                                         const float *__restrict__ seg_d,
-                                                         const uint32_t *__restrict__ seg_i,
-                                                         float *__restrict__ out_d,
-                                                         int64_t *__restrict__ out_i) {
+__global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dist, long n,
+                                                   long nseg, long stride, long seg0, uint32_t id0,
+                                                   float *__restrict__ seg_d,
+                                                   uint32_t *__restrict__ seg_i) {
+  constexpr int PL = kTopkSub / 64;  // columns per lane and sub-block
+  __shared__ float c_d[4][kTopkCap];
+  __shared__ uint32_t c_i[4][kTopkCap];
   const int lane = threadIdx.x & 63;
-  const long row = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (row >= m) return;
-  float td[K];
-  uint32_t ti[K];
-  topk_clear<K>(td, ti);
-  for (int l = lane; l < nseg; l += 64) {
-    const long base = (row * nseg + l) * K;
-    take_sorted<K>(td, ti, seg_d + base, seg_i + base);
+  const int w = threadIdx.x >> 6;
+  const long row = blockIdx.y;
+  const float *p = dist + row * n;
+  auto load_sub = [&](float (&v)[PL], long cb) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < PL; ++u) {
+      const long j = cb + u * 64 + lane;
+      v[u] = p[j < n ? j : n - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < PL; ++u) {
+      const long j = cb + u * 64 + lane;
+      if (j >= n) v[u] = __builtin_inff();
+    }
+  };
+  for (long seg = (long)blockIdx.x * 4 + w; seg < nseg; seg += (long)gridDim.x * 4) {
+    float *od = seg_d + (row * stride + seg0 + seg) * K;
+    uint32_t *oi = seg_i + (row * stride + seg0 + seg) * K;
+    if (n <= 0) {  // no columns (dist may be null): an empty list
+      if (lane < K) {
+        od[lane] = __builtin_inff();
+        oi[lane] = 0xFFFFFFFFu;
+      }
+      continue;
+    }
+    const long c0 = seg * kTopkSeg;
+    float v[PL];
+    load_sub(v, c0);
+    // 1. lane minima over the first sub-block, then thr.
+    float md = __builtin_inff();
+    uint32_t mi = 0xFFFFFFFFu;
+#pragma unroll
+    for (int u = 0; u < PL; ++u) {
+      const long j = c0 + u * 64 + lane;
+      const uint32_t id = j < n ? (uint32_t)j + id0 : 0xFFFFFFFFu;
+      if (lex_less(v[u], id, md, mi)) {
+        md = v[u];
+        mi = id;
+      }
+    }
+    float thr_d = __builtin_inff();
+    uint32_t thr_i = 0xFFFFFFFFu;
+    for (int r = 0; r < K; ++r) {
+      float bd = md;
+      uint32_t bi = mi;
+      wave_lex_min(bd, bi);
+      thr_d = bd;
+      thr_i = bi;
+      if (md == bd && mi == bi) {
+        md = __builtin_inff();
+        mi = 0xFFFFFFFFu;
+      }
+    }
+    // 2. pack the entries at or below thr, sub-block by sub-block.
+    int cnt = 0;
+    for (int sb = 0; sb < kTopkSeg / kTopkSub; ++sb) {
+      const long cb = c0 + (long)sb * kTopkSub;
+      if (cb >= n) break;  // wavefront-uniform
+      if (sb > 0) load_sub(v, cb);
+#pragma unroll
+      for (int u = 0; u < PL; ++u) {
+        const long j = cb + u * 64 + lane;
+        const uint32_t id = (uint32_t)j + id0;
+        const bool cand = j < n && !lex_less(thr_d, thr_i, v[u], id);
+        const uint64_t b = __ballot(cand);
+        const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi(
+                                  (uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+        if (cand && pos < kTopkCap) {
+          c_d[w][pos] = v[u];
+          c_i[w][pos] = id;
+        }
+        cnt += __popcll(b);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (cnt <= kTopkCap) {
+      // 3. four candidates per lane, sorted, then K pops.
+      float td[4];
+      uint32_t ti[4];
+      topk_clear<4>(td, ti);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int q = lane + 64 * t;
+        if (q < cnt) topk_insert<4>(td, ti, c_d[w][q], c_i[w][q]);
+      }
+      wave_pop_list<4, K>(td, ti, od, oi, lane);
+    } else {
+      float td[K];
+      uint32_t ti[K];
+      topk_clear<K>(td, ti);
+      for (int sb = 0; sb < kTopkSeg / kTopkSub; ++sb) {
+        const long cb = c0 + (long)sb * kTopkSub;
+        if (cb >= n) break;
+        load_sub(v, cb);
+#pragma unroll
+        for (int u = 0; u < PL; ++u) {
+          const long j = cb + u * 64 + lane;
+          const uint32_t id = (uint32_t)j + id0;
+          if (j < n && !lex_less(thr_d, thr_i, v[u], id)) topk_insert<K>(td, ti, v[u], id);
+        }
+      }
+      wave_pop_list<K, K>(td, ti, od, oi, lane);
+    }
+    // the next segment's stores to this LDS list must follow these reads
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
-  wave_pop_topk<K>(td, ti, k_out, lane, out_d + row * k_out, out_i + row * k_out);
 }
 
-// Folds lists [l0, l1) of each row (`stride` lists per row) into one: one
-// wavefront per row takes them into lane lists, then pops either the row's
-// best k_out into out_d / out_i (when out_d is set) or its best K back into
-// list 0 as a sorted list. Only lane 0 reads list 0, before any pop writes it.
+// Stage 2: merges lists [l0, l1) of each row (`stride` sorted lists of K
+// per row), one wavefront per row, the same way stage 1 treats columns:
+// thr = the K-th smallest of the 64 lanes' smallest list heads (K distinct
+// entries, so the row's K best lie at or below it); the entries at or below
+// thr (a walk of each list that starts at or below thr) are packed into an
+// LDS list by an LDS counter, four per lane are sorted and K wavefront
+// pops give the result: the row's best k_out into out_d / out_i (when out_d
+// is set, ids ~0 -> -1) or its best K back into list 0 as a sorted list.
+// More than kTopkCap such entries: every lane folds its lists' entries at or
+// below thr into a list of K instead (exact either way). Only lane 0 reads
+// list 0 and it does before any pop writes it.
 template <int K>
 __global__ __launch_bounds__(256) void k_topk_rows_fold(long m, long stride, long l0, long l1,
                                                         int k_out, float *__restrict__ seg_d,
                                                         uint32_t *__restrict__ seg_i,
                                                         float *__restrict__ out_d,
                                                         int64_t *__restrict__ out_i) {
+  __shared__ float c_d[4][kTopkCap];
+  __shared__ uint32_t c_i[4][kTopkCap];
+  __shared__ int c_n[4];
   const int lane = threadIdx.x & 63;
-  const long row = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (row >= m) return;
-  float td[K];
-  uint32_t ti[K];
-  topk_clear<K>(td, ti);
-  for (long l = l0 + lane; l < l1; l += 64) {
-    const long base = (row * stride + l) * K;
-    take_sorted<K>(td, ti, seg_d + base, seg_i + base);
-  }
-  if (out_d) {
-    wave_pop_topk<K>(td, ti, k_out, lane, out_d + row * k_out, out_i + row * k_out);
-    return;
-  }
-  const long base = row * stride * K;
-  for (int k = 0; k < K; ++k) {
-    float bd = td[0];
-    uint32_t bi = ti[0];
-    for (int o = 32; o > 0; o >>= 1) {
-      const float od = __shfl_xor(bd, o);
-      const uint32_t oi = __shfl_xor(bi, o);
-      if (lex_less(od, oi, bd, bi)) {
-        bd = od;
-        bi = oi;
-      }
-    }
-    if (lane == 0) {
-      seg_d[base + k] = bd;
-      seg_i[base + k] = bi;
-    }
-    if (td[0] == bd && ti[0] == bi) {
+  const int w = threadIdx.x >> 6;
+  const long row = (long)blockIdx.x * 4 + w;
+  if (row >= m) return;  // no barrier below
+  const float *rd = seg_d + row * stride * K;
+  const uint32_t *ri = seg_i + row * stride * K;
+  const long nl = l1 - l0;
+  // 1. the lane's smallest head; thr.
+  float md = __builtin_inff();
+  uint32_t mi = 0xFFFFFFFFu;
+  for (long b = 0; b < nl; b += 64 * 8) {
+    float hd[8];
+    uint32_t hi[8];
 #pragma unroll
-      for (int t = 0; t + 1 < K; ++t) {
-        td[t] = td[t + 1];
-        ti[t] = ti[t + 1];
+    for (int t = 0; t < 8; ++t) {
+      const long l = b + t * 64 + lane;
+      const long lc = l0 + (l < nl ? l : nl - 1);
+      hd[t] = rd[lc * K];
+      hi[t] = ri[lc * K];
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const long l = b + t * 64 + lane;
+      if (l < nl && lex_less(hd[t], hi[t], md, mi)) {
+        md = hd[t];
+        mi = hi[t];
       }
-      td[K - 1] = __builtin_inff();
-      ti[K - 1] = 0xFFFFFFFFu;
+    }
+  }
+  float thr_d = __builtin_inff();
+  uint32_t thr_i = 0xFFFFFFFFu;
+  for (int r = 0; r < K; ++r) {
+    float bd = md;
+    uint32_t bi = mi;
+    wave_lex_min(bd, bi);
+    thr_d = bd;
+    thr_i = bi;
+    if (md == bd && mi == bi) {
+      md = __builtin_inff();
+      mi = 0xFFFFFFFFu;
+    }
+  }
+  // 2. entries at or below thr.
+  if (lane == 0) c_n[w] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (long l = lane; l < nl; l += 64) {
+    const float *ld = rd + (l0 + l) * K;
+    const uint32_t *li = ri + (l0 + l) * K;
+    for (int k = 0; k < K; ++k) {
+      const float d = ld[k];
+      const uint32_t id = li[k];
+      if (lex_less(thr_d, thr_i, d, id) || id == 0xFFFFFFFFu) break;  // sorted: the rest is above
+      const int pos = atomicAdd(&c_n[w], 1);
+      if (pos < kTopkCap) {
+        c_d[w][pos] = d;
+        c_i[w][pos] = id;
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const int cnt = c_n[w];
+  if (cnt <= kTopkCap) {
+    float td[4];
+    uint32_t ti[4];
+    topk_clear<4>(td, ti);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int q = lane + 64 * t;
+      if (q < cnt) topk_insert<4>(td, ti, c_d[w][q], c_i[w][q]);
+    }
+    if (out_d) {
+      wave_pop_topk<4>(td, ti, k_out, lane, out_d + row * k_out, out_i + row * k_out);
+    } else {
+      wave_pop_list<4, K>(td, ti, seg_d + row * stride * K, seg_i + row * stride * K, lane);
+    }
+  } else {
+    float td[K];
+    uint32_t ti[K];
+    topk_clear<K>(td, ti);
+    for (long l = lane; l < nl; l += 64)
+      take_sorted<K>(td, ti, rd + (l0 + l) * K, ri + (l0 + l) * K);
+    if (out_d) {
+      wave_pop_topk<K>(td, ti, k_out, lane, out_d + row * k_out, out_i + row * k_out);
+    } else {
+      wave_pop_list<K, K>(td, ti, seg_d + row * stride * K, seg_i + row * stride * K, lane);
     }
   }
 }
@@ -642,14 +779,9 @@ hipError_t launch_search(const SearchJobDev *d_jobs, const SearchItem *d_items,
 }
 
 long topk_rows_segments(long m, long n) {
-  // About 8192 wavefronts over the matrix, each segment at least 2048
-  // columns (so m * segments < 8192 + m).
-  long seg = (8192 + m - 1) / m;
-  const long max_seg = (n + 2047) / 2048;
-  if (seg > max_seg) seg = max_seg;
-  if (seg < 1) seg = 1;
-  if (seg > 65535) seg = 65535;
-  return seg;
+  (void)m;
+  const long seg = (n + kTopkSeg - 1) / kTopkSeg;  // one wavefront per kTopkSeg columns
+  return seg > 1 ? seg : 1;
 }
 
 hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out_d,
@@ -657,13 +789,13 @@ hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out
   if (m <= 0) return hipSuccess;
   if (k < 1 || k > kSearchMaxK || m > 65535 || n >= 0xFFFFFFFFL) return hipErrorInvalidValue;
   const long nseg = topk_rows_segments(m, n);
-  const long span = (n + nseg - 1) / nseg;
-  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3((unsigned)((nseg + 3) / 4), (unsigned)m),
-                     dim3(256), 0, st, dist, n, span, nseg, nseg, 0L, 0u, seg_d, seg_i);
+  const long gx = (nseg + 3) / 4 < 65535 ? (nseg + 3) / 4 : 65535;
+  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3((unsigned)gx, (unsigned)m), dim3(256), 0, st,
+                     dist, n, nseg, nseg, 0L, 0u, seg_d, seg_i);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_topk_rows_merge<kSearchMaxK>, dim3((unsigned)((m + 3) / 4)), dim3(256), 0,
-                     st, m, (int)nseg, k, seg_d, seg_i, out_d, out_i);
+  hipLaunchKernelGGL(k_topk_rows_fold<kSearchMaxK>, dim3((unsigned)((m + 3) / 4)), dim3(256), 0,
+                     st, m, nseg, 0L, nseg, k, seg_d, seg_i, out_d, out_i);
   return hipGetLastError();
 }
 
@@ -672,9 +804,9 @@ hipError_t launch_topk_rows_chunk(const float *dist, long m, long n, long stride
   if (m <= 0 || n <= 0) return hipSuccess;
   const long nseg = topk_rows_segments(m, n);
   if (m > 65535 || n >= 0xFFFFFFFFL || seg0 + nseg > stride) return hipErrorInvalidValue;
-  const long span = (n + nseg - 1) / nseg;
-  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3((unsigned)((nseg + 3) / 4), (unsigned)m),
-                     dim3(256), 0, st, dist, n, span, nseg, stride, seg0, id0, seg_d, seg_i);
+  const long gx = (nseg + 3) / 4 < 65535 ? (nseg + 3) / 4 : 65535;
+  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3((unsigned)gx, (unsigned)m), dim3(256), 0, st,
+                     dist, n, nseg, stride, seg0, id0, seg_d, seg_i);
   return hipGetLastError();
 }
 
