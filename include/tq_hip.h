@@ -190,8 +190,12 @@ typedef struct tq_search_plan_s *tq_search_plan;
 /* Builds a reusable search over `jobs` (copied): checks every job, sizes the
  * work queue and allocates all device scratch once. mode TQ_MODE_MFMA: one
  * persistent launch over a (query tile, code range) queue with the top-k
- * fused into the scan (k <= TQ_SEARCH_MAX_K). TQ_MODE_FP32: per job the plain
- * fp32 distance matrix in row blocks, then a row top-k. `tuning` may be NULL. */
+ * fused into the scan (k <= TQ_SEARCH_MAX_K); queries aimed at a collection
+ * of 2,000,000 codes or more run blocked instead (MFMA distances of a chunk
+ * of codes into a scratch block of at most 256 MB, a row top-k per chunk,
+ * candidates folded per query), with the same results. TQ_MODE_FP32: per
+ * job the plain fp32 distance matrix in row blocks, then a row top-k.
+ * `tuning` may be NULL. */
 int tq_search_plan_create(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
                           const tq_search_tuning *tuning, tq_search_plan *out);
 /* Enqueues the plan on `stream`: query preparation, the scan and the top-k.
