@@ -344,15 +344,14 @@ __global__ __launch_bounds__(LutShape<D>::THREADS) void k_lut_nib(SpaceDev s, co
   }
 }
 
+// Compute units of the current device (asked per launch: cheap, and correct
+// for every device and thread).
 int cu_count() {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0)
-      cus = 256;
-  }
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
   return cus;
 }
 

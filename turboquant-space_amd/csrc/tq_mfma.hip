@@ -94,24 +94,31 @@ struct MfmaShape {
 };
 
 // Two workgroups per CU (two wavefronts per SIMD) so one wavefront's MFMA
-// chain overlaps the other's decode, LDS traffic and barrier waits.
-template <bool NIB, bool LO>
+// chain overlaps the other's decode, LDS traffic and barrier waits. QT = 64
+// queries per workgroup (2 x 2 blocks of 32 x 32 per wavefront) or 32 (1 x 2
+// blocks) for batches of at most 32 queries: every output element depends
+// only on its own query row and code column, so both give the same bits.
+template <bool NIB, bool LO, int QT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma(SpaceDev s, const _Float16 *__restrict__ planes,
                                               const float *__restrict__ prep, long m,
                                               const uint8_t *__restrict__ codes, long n,
                                               float *__restrict__ out) {
   using Sh = MfmaShape<NIB>;
+  static_assert(QT == 32 || QT == 64, "query tile");
+  constexpr int RB = QT / 32;                                  // 32-row blocks per wavefront
   constexpr int CROW = Sh::CROW, CPARTS = Sh::CPARTS;
-  constexpr int ALOADS = kPlanes * kTQ * (kKC / 8) / 256;  // 16-byte A loads per thread
+  constexpr int AUNITS = kPlanes * QT * (kKC / 8);             // 16-byte A units per chunk
+  constexpr int ALOADS = (AUNITS + 255) / 256;                 // per thread (QT = 32: the
+                                                               // last unit is loaded twice)
   constexpr int CLOADS = kTN * CPARTS / 256;         // 16-byte code loads per thread
-  __shared__ _Float16 a_lds[2][kPlanes][kTQ][kARow];
+  __shared__ _Float16 a_lds[2][kPlanes][QT][kARow];
   __shared__ uint32_t c_lds[2][kTN * CROW / 4];
   __shared__ uint32_t l_tab[256];
-  __shared__ float l_qn[kTQ][2];  // q_norm_sq, q_norm of the tile's queries
+  __shared__ float l_qn[QT][2];  // q_norm_sq, q_norm of the tile's queries
 
   const int tid = threadIdx.x;
-  if (tid < kTQ) {
-    const long q = (long)blockIdx.y * kTQ + tid;
+  if (tid < QT) {
+    const long q = (long)blockIdx.y * QT + tid;
     const float *pq = prep + (q < m ? q : 0L) * prep_stride(s.D);
     l_qn[tid][0] = pq[2 * s.D];
     l_qn[tid][1] = pq[2 * s.D + 1];
@@ -144,7 +151,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int lr = lane & 31, kg = lane >> 5;
-  const long q0 = (long)blockIdx.y * kTQ;
+  const long q0 = (long)blockIdx.y * QT;
   const long j0 = (long)blockIdx.x * kTN;
   const long mD = m * (long)s.D;
   const int nchunks = s.D / kKC;
@@ -159,13 +166,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   int adst[ALOADS];
 #pragma unroll
   for (int i = 0; i < ALOADS; ++i) {
-    const int idx = i * 256 + tid;
-    const int p = idx / (kTQ * kKC / 8);
-    const int row = (idx / (kKC / 8)) % kTQ;
+    const int idx = i * 256 + tid < AUNITS ? i * 256 + tid : AUNITS - 1;
+    const int p = idx / (QT * kKC / 8);
+    const int row = (idx / (kKC / 8)) % QT;
     const int part = idx % (kKC / 8);
     const long q = q0 + row;
     asrc[i] = planes + p * mD + (q < m ? q : 0L) * s.D + 8 * part;
-    adst[i] = (p * kTQ + row) * kARow + 8 * part;
+    adst[i] = (p * QT + row) * kARow + 8 * part;
   }
   const uint8_t *csrc[CLOADS];
   int cdst[CLOADS];
@@ -207,9 +214,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     for (int i = 0; i < CLOADS; ++i) *reinterpret_cast<u32x4a4 *>(&c_lds[b][cdst[i]]) = ring_c[k][i];
   };
 
-  f16v acc_m[2][2], acc_q[2][2];
+  f16v acc_m[RB][2], acc_q[RB][2];
 #pragma unroll
-  for (int r = 0; r < 2; ++r)
+  for (int r = 0; r < RB; ++r)
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -227,9 +234,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int b = ch & 1;
 #pragma unroll
     for (int kb = 0; kb < kKC / 16; ++kb) {
-      h8 a[2][kPlanes];
+      h8 a[RB][kPlanes];
 #pragma unroll
-      for (int r = 0; r < 2; ++r)
+      for (int r = 0; r < RB; ++r)
 #pragma unroll
         for (int p = 0; p < kPlanes; ++p)
           a[r][p] = *reinterpret_cast<const h8 *>(&a_lds[b][p][32 * r + lr][kb * 16 + 8 * kg]);
@@ -254,7 +261,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       // Each pass over all four 32x32 blocks before the next, so consecutive
       // MFMAs never wait on the accumulator they just wrote.
 #define TQ_PASS(ACC, A, B)                                                               \
-  _Pragma("unroll") for (int r = 0; r < 2; ++r)                                          \
+  _Pragma("unroll") for (int r = 0; r < RB; ++r)                                         \
       _Pragma("unroll") for (int c = 0; c < 2; ++c) ACC[r][c] =                          \
           __builtin_amdgcn_mfma_f32_32x32x16_f16(a[r][A], B[c], ACC[r][c], 0, 0, 0);
       TQ_PASS(acc_m, 0, bh)
@@ -313,7 +320,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const float fq = s.scale * gamma * kQjlUnscale;
     const float xns = xn * xn;
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int r = 0; r < RB; ++r)
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int row = 32 * r + 8 * (v >> 2) + 4 * kg + (v & 3);
@@ -342,13 +349,24 @@ hipError_t launch_mfma_planes(const SpaceDev &s, const _Float16 *planes, const f
                               const uint8_t *codes, long n, float *out, hipStream_t st) {
   if (s.D < kKC || m > 65535L * kTQ) return hipErrorNotSupported;
   if (m <= 0 || n <= 0) return hipSuccess;
-  dim3 grid((unsigned)((n + kTN - 1) / kTN), (unsigned)((m + kTQ - 1) / kTQ));
+  const unsigned gx = (unsigned)((n + kTN - 1) / kTN);
+  if (m <= 32) {  // one 32-query tile: half the matrix work of a 64-query one
+    const dim3 grid(gx, 1);
+    if (s.nib && s.mfma_lo)
+      hipLaunchKernelGGL((k_mfma<true, true, 32>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+    else if (s.nib)
+      hipLaunchKernelGGL((k_mfma<true, false, 32>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+    else
+      hipLaunchKernelGGL((k_mfma<false, true, 32>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+    return hipGetLastError();
+  }
+  const dim3 grid(gx, (unsigned)((m + kTQ - 1) / kTQ));
   if (s.nib && s.mfma_lo)
-    hipLaunchKernelGGL((k_mfma<true, true>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+    hipLaunchKernelGGL((k_mfma<true, true, kTQ>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
   else if (s.nib)
-    hipLaunchKernelGGL((k_mfma<true, false>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+    hipLaunchKernelGGL((k_mfma<true, false, kTQ>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
   else
-    hipLaunchKernelGGL((k_mfma<false, true>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+    hipLaunchKernelGGL((k_mfma<false, true, kTQ>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
   return hipGetLastError();
 }
 
