@@ -5,9 +5,10 @@ kernel.
 
 Bars: the plan's top-k equals the (distance, id)-ordered top-k of the full
 MFMA distance matrix, distances bit for bit, with one chunk and with several
-(m = 5, 37, 100 over 2.3M codes: 1, 2 and 4 chunks); a plan holding a blocked
-and a fused batch gives each job what it gets alone; graph replays equal eager
-runs; the server over such a collection returns what a one-shot plan returns.
+(m = 5, 37, 100 over 2.3M codes: 1, 2 and 4 chunks), for 4-bit and 8-bit
+codes; a plan holding a blocked and a fused batch gives each job what it gets
+alone; graph replays equal eager runs; the server over such a collection
+returns what a one-shot plan returns.
 """
 import sys
 
@@ -94,6 +95,27 @@ def test_blocked_equals_topk_of_matrix(gpu, big, m):
                                 graph_replays=2 if m == 37 else 0)
     np.testing.assert_array_equal(got_i, want_i)
     np.testing.assert_array_equal(got_d, want_d)
+
+
+@pytest.mark.parametrize("m", [9, 70])
+def test_blocked_byte_codes(gpu, m):
+    # 8-bit codes (the MFMA form's c_lo pass), 32- and 64-query tiles
+    import torch
+    capi = gpu
+    say(f"[gpu] blocked search, 8-bit codes, m={m}")
+    sp = capi.Space(128, 8)
+    n = 2_100_000
+    g = torch.Generator(device="cuda").manual_seed(11)
+    codes = torch.empty((n, sp.cs), dtype=torch.uint8, device="cuda")
+    for s0 in range(0, n, 700_000):
+        x = torch.randn((700_000, 128), generator=g, device="cuda")
+        sp.encode(x, 700_000, codes[s0:s0 + 700_000])
+    Qd = torch.randn((m, 128), generator=g, device="cuda")
+    want_d, want_i = matrix_topk(torch, capi, sp, Qd, m, codes, n, K)
+    (got_d, got_i), = plan_topk(torch, capi, [(sp, Qd, m, codes, n)], K)
+    np.testing.assert_array_equal(got_i, want_i)
+    np.testing.assert_array_equal(got_d, want_d)
+    sp.close()
 
 
 def test_mixed_blocked_and_fused(gpu, big):
