@@ -155,6 +155,20 @@ int tq_distance_symmetric_prepared(tq_space s, const float *prepared,
 int tq_rotated_coords_batch(tq_space s, const float *X, uint64_t n, float *out,
                             int mem, void *stream);
 
+/* Encode graphs -- no counterpart in the reference (encodeBatch,
+ * space_turbo_quant.h:400-407, is an OpenMP loop on the host). The encode of
+ * a fixed batch of n rows at X (device memory, n x dim float32) into codes
+ * (device memory, n x code_size), captured once into a HIP graph; each
+ * tq_encode_graph_run replays it on `stream` with one graph launch and reads
+ * X when it runs. For ingest loops and serving appends that refill the same
+ * staging buffer. X and codes must stay allocated for the graph's life; host
+ * memory is rejected (TQ_EINVAL). */
+typedef struct tq_encode_graph_s *tq_encode_graph;
+int tq_encode_graph_create(tq_space s, const float *X, uint64_t n, uint8_t *codes,
+                           tq_encode_graph *out);
+int tq_encode_graph_run(tq_encode_graph g, void *stream);
+int tq_encode_graph_destroy(tq_encode_graph g);
+
 /* ---- top-k search ---------------------------------------------------------
  * No reference counterpart: the reference returns full distance matrices and
  * its users rank rows with numpy (argpartition + argsort,

@@ -391,6 +391,87 @@ int tq_encode_batch(tq_space s, const float *X, uint64_t n, uint8_t *codes, int 
   return stage_out(outb, mem, st, what);
 }
 
+struct tq_encode_graph_s {
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~tq_encode_graph_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace {
+// True when `p` is device (or managed) memory of `device`, so a kernel
+// replayed from a graph can read it; host memory is not.
+bool on_device(const void *p, int device) {
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) &&
+         (a.type == hipMemoryTypeManaged || a.device == device);
+}
+}  // namespace
+
+int tq_encode_graph_create(tq_space s, const float *X, uint64_t n, uint8_t *codes,
+                           tq_encode_graph *out) {
+  const char *what = "tq_encode_graph_create";
+  if (!out) return fail(TQ_EINVAL, "%s: out is NULL", what);
+  *out = nullptr;
+  if (!s) return fail(TQ_EINVAL, "%s: NULL space", what);
+  if (n == 0 || n > (uint64_t)INT64_MAX / 4096) return fail(TQ_EINVAL, "%s: bad n", what);
+  if (!X || !codes) return fail(TQ_EINVAL, "%s: NULL buffer", what);
+  if (int r = select_device(s, what)) return r;
+  if (!on_device(X, s->device) || !on_device(codes, s->device))
+    return fail(TQ_EINVAL, "%s: X and codes must be device memory of the space's device", what);
+  hipStream_t cap = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, what);
+  hipGraph_t g = nullptr;
+  e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    const hipError_t le = tq::launch_encode(s->dev, X, (long)n, codes, cap);
+    e = hipStreamEndCapture(cap, &g);
+    if (le != hipSuccess) e = le;
+  }
+  (void)hipStreamDestroy(cap);
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return hip_fail(e, what);
+  }
+  auto *eg = new (std::nothrow) tq_encode_graph_s;
+  if (!eg) {
+    (void)hipGraphDestroy(g);
+    return fail(TQ_ENOMEM, "%s: out of host memory", what);
+  }
+  eg->device = s->device;
+  eg->graph = g;
+  if ((e = hipGraphInstantiate(&eg->exec, g, nullptr, nullptr, 0)) != hipSuccess) {
+    eg->exec = nullptr;
+    delete eg;
+    return hip_fail(e, what);
+  }
+  *out = eg;
+  return TQ_OK;
+}
+
+int tq_encode_graph_run(tq_encode_graph g, void *stream) {
+  const char *what = "tq_encode_graph_run";
+  if (!g) return fail(TQ_EINVAL, "%s: NULL graph", what);
+  hipError_t e = hipSetDevice(g->device);
+  if (e == hipSuccess) e = hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? TQ_OK : hip_fail(e, what);
+}
+
+int tq_encode_graph_destroy(tq_encode_graph g) {
+  if (!g) return TQ_OK;
+  (void)hipSetDevice(g->device);
+  delete g;
+  return TQ_OK;
+}
+
 int tq_prepare_queries(tq_space s, const float *Q, uint64_t m, float *prepared, int mem,
                        void *stream) {
   const char *what = "tq_prepare_queries";

@@ -31,7 +31,8 @@ EXPORTED = [
     "tq_space_info_get", "tq_space_tables", "tq_encode_batch", "tq_prepare_queries",
     "tq_distance_1_to_n", "tq_distance_m_to_n", "tq_distance_prepared",
     "tq_distance_m_to_n_symmetric", "tq_distance_m_to_n_symmetric_ex", "tq_prepare_symmetric",
-    "tq_distance_symmetric_prepared", "tq_rotated_coords_batch", "tq_search_plan_create", "tq_search_plan_run",
+    "tq_distance_symmetric_prepared", "tq_rotated_coords_batch", "tq_encode_graph_create",
+    "tq_encode_graph_run", "tq_encode_graph_destroy", "tq_search_plan_create", "tq_search_plan_run",
     "tq_search_plan_run_graph", "tq_search_plan_tune", "tq_search_plan_stats",
     "tq_search_plan_destroy", "tq_search", "tq_topk_rows", "tq_server_create",
     "tq_server_destroy", "tq_server_add_collection", "tq_server_submit_append",
@@ -155,6 +156,12 @@ def lib():
         L.tq_search.argtypes = [JP, U64, I, I, P]
         L.tq_topk_rows.restype = I
         L.tq_topk_rows.argtypes = [P, U64, U64, I, P, P, P]
+        L.tq_encode_graph_create.restype = I
+        L.tq_encode_graph_create.argtypes = [P, P, U64, P, ctypes.POINTER(P)]
+        L.tq_encode_graph_run.restype = I
+        L.tq_encode_graph_run.argtypes = [P, P]
+        L.tq_encode_graph_destroy.restype = I
+        L.tq_encode_graph_destroy.argtypes = [P]
         U32 = ctypes.c_uint32
         L.tq_server_create.restype = I
         L.tq_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]
@@ -265,6 +272,31 @@ def make_jobs(jobs):
         arr[i] = SearchJob(sp.h.value if isinstance(sp, Space) else sp, ptr(q), m, ptr(c), n,
                            ptr(od), ptr(oi))
     return arr
+
+
+class EncodeGraph:
+    """Owns a tq_encode_graph: the encode of n rows at X into codes (device
+    buffers), replayed by run(); each replay reads X when it runs."""
+
+    def __init__(self, space, X, n, codes):
+        self._keep = (space, X, codes)  # must outlive the graph
+        h = ctypes.c_void_p()
+        check(lib().tq_encode_graph_create(space.h, ptr(X), n, ptr(codes), ctypes.byref(h)))
+        self.h = h
+
+    def run(self, stream=None):
+        check(lib().tq_encode_graph_run(self.h, stream))
+
+    def close(self):
+        if self.h:
+            lib().tq_encode_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class SearchPlan:
