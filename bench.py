@@ -10,7 +10,10 @@ All inputs are resident in HBM before the timed region.
 Printed: one JSON line. `value` is distances/s of the fastest path that meets
 its accuracy bar (fp16 hi+lo MFMA, <= 1e-4 of the fp64 estimator, when built;
 else the plain fp32 path); the plain fp32 path, distance_1_to_n and encode are
-reported beside it, with the reference's own CPU build timed on this host.
+reported beside it, with the reference's own CPU build timed on this host,
+followed by the recall, symmetric (dim 1000, 4,096 x 2M codes), batched
+(256 uneven batches, 5 collections) and serving (fixed-QPS) measurements,
+each skippable by a --no-* flag.
 """
 import argparse
 import ctypes
@@ -35,8 +38,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def gaussian_mixture(torch, n, dim, k, std, seed, centres=None, chunk=1_000_000):
-    """Rows of gm(n, dim, k, std, normalised) generated on the GPU in chunks."""
+def gaussian_mixture(torch, n, dim, k, std, seed, centres=None, chunk=1_000_000, normalise=True):
+    """Rows of gm(n, dim, k, std) generated on the GPU in chunks, L2-normalised
+    unless normalise is False."""
     g = torch.Generator(device="cuda").manual_seed(seed)
     if centres is None:
         centres = torch.randn((k, dim), generator=g, device="cuda")
@@ -44,7 +48,8 @@ def gaussian_mixture(torch, n, dim, k, std, seed, centres=None, chunk=1_000_000)
         rows = min(chunk, n - start)
         lab = torch.randint(0, k, (rows,), generator=g, device="cuda")
         x = centres[lab] + std * torch.randn((rows, dim), generator=g, device="cuda")
-        x = x / torch.linalg.vector_norm(x, dim=1, keepdim=True)
+        if normalise:
+            x = x / torch.linalg.vector_norm(x, dim=1, keepdim=True)
         yield start, x.contiguous(), centres
 
 
@@ -279,6 +284,47 @@ RECALL_N = 1_000_000
 RECALL_Q = 256
 
 
+# Symmetric workload: dim 1000 (padded to 1024), 4-bit; 2,000,000 base
+# vectors and 4,096 query vectors of one 4,096-centre mixture (std 0.5, not
+# normalised; seeds 4 and 5).
+SYM_DIM, SYM_BITS, SYM_N, SYM_M = 1000, 4, 2_000_000, 4096
+
+
+def bench_symmetric(torch, capi, stream, sh, n=SYM_N, m=SYM_M, rows=512):
+    """encode_batch of the 2M base vectors, then distance_m_to_n_symmetric of
+    the 4,096 query codes against them (the centroid-product LUT scan), in
+    blocks of `rows` query codes into one reused [rows, n] output."""
+    sp = capi.Space(SYM_DIM, SYM_BITS)
+    codes_b = torch.empty((n, sp.cs), dtype=torch.uint8, device="cuda")
+    centres = None
+    enc_ms = 0.0
+    for start, x, centres in gaussian_mixture(torch, n, SYM_DIM, 4096, 0.5, 4, normalise=False):
+        r = x.shape[0]
+        enc_ms += event_time(torch, stream,
+                             lambda: sp.encode(x, r, codes_b[start:start + r], stream=sh), 1)
+    codes_a = torch.empty((m, sp.cs), dtype=torch.uint8, device="cuda")
+    for start, x, _ in gaussian_mixture(torch, m, SYM_DIM, 4096, 0.5, 5, centres=centres,
+                                        normalise=False):
+        sp.encode(x, x.shape[0], codes_a[start:start + x.shape[0]], stream=sh)
+    out = torch.empty((rows, n), dtype=torch.float32, device="cuda")
+
+    def sweep():
+        for r0 in range(0, m, rows):
+            rc = min(rows, m - r0)
+            sp.symmetric(codes_a[r0:r0 + rc], rc, codes_b, n, out, stream=sh)
+
+    ms = event_time(torch, stream, sweep, 1)
+    res = {"dim": SYM_DIM, "bits": SYM_BITS, "codes_b": n, "codes_a": m,
+           "encode_vec_per_s": n / (enc_ms / 1e3),
+           "symmetric_ms": ms, "dist_per_s": m * n / (ms / 1e3),
+           "code_bytes_per_s": m * n * sp.cs / (ms / 1e3)}
+    log(f"symmetric {m} x {n} (dim {SYM_DIM}, {SYM_BITS}-bit): {ms:.1f} ms, "
+        f"{res['dist_per_s'] / 1e9:.2f} G dist/s; encode {res['encode_vec_per_s'] / 1e6:.1f} M vec/s")
+    del out, codes_a, codes_b
+    torch.cuda.empty_cache()
+    return res
+
+
 def bench_recall(torch, capi, sh, n=RECALL_N, m=RECALL_Q, k=10):
     """Recall@1 and recall@10 of the quantized searches against the exact
     neighbours (squared L2 over the raw vectors, fp64 brute force on the GPU)
@@ -371,6 +417,8 @@ def main():
                     help="skip the batched uneven-launch measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-recall", action="store_true", help="skip the recall measurement")
+    ap.add_argument("--no-symmetric", action="store_true",
+                    help="skip the symmetric (dim 1000, 4,096 x 2M) measurement")
     ap.add_argument("--serve-seconds", type=float, default=12.0)
     ap.add_argument("--qps-mfma", type=float, default=40.0)
     ap.add_argument("--qps-fp32", type=float, default=4.0)
@@ -485,6 +533,7 @@ def main():
     del codes, out, one
     torch.cuda.empty_cache()
     recall = None if args.no_recall else bench_recall(torch, capi, sh)
+    symmetric = None if args.no_symmetric else bench_symmetric(torch, capi, stream, sh)
     batched = None if args.no_batched else bench_batched(torch, capi, stream, sh)
     serving = None
     if not args.no_serving:
@@ -507,6 +556,7 @@ def main():
         "distance_1_to_n": one_to_n,
         "encode_vec_per_s": enc_vps,
         "recall": recall,
+        "symmetric": symmetric,
         "batched_uneven": batched,
         "serving": serving,
     }
