@@ -11,7 +11,8 @@ Printed: one JSON line. `value` is distances/s of the fastest path that meets
 its accuracy bar (fp16 hi+lo MFMA, <= 1e-4 of the fp64 estimator, when built;
 else the plain fp32 path); the plain fp32 path, distance_1_to_n and encode are
 reported beside it, with the reference's own CPU build timed on this host,
-followed by the recall, symmetric (dim 1000, 4,096 x 2M codes), batched
+followed by the recall, symmetric (dim 1000, 4,096 x 2M codes), 8-bit
+(dim 768, 1,024 x 1M codes, fp32 and MFMA), batched
 (256 uneven batches, 5 collections) and serving (fixed-QPS) measurements,
 each skippable by a --no-* flag.
 """
@@ -325,6 +326,46 @@ def bench_symmetric(torch, capi, stream, sh, n=SYM_N, m=SYM_M, rows=512):
     return res
 
 
+# Byte-code workload: dim 768, 8-bit (1,036-byte codes); 1,000,000 base and
+# 1,024 query vectors of one 1,024-centre mixture (std 0.3, L2-normalised;
+# seeds 2 and 3).
+A8_DIM, A8_BITS, A8_N, A8_M = 768, 8, 1_000_000, 1024
+
+
+def bench_asym8(torch, capi, stream, sh, n=A8_N, m=A8_M):
+    """encode_batch of the 1M base vectors, then distance_m_to_n of the 1,024
+    queries against them (output [1024, 1M] in HBM) by the plain fp32 path and
+    by the MFMA form (hi+lo fp16 queries, c_hi + c_lo centroids)."""
+    sp = capi.Space(A8_DIM, A8_BITS)
+    codes = torch.empty((n, sp.cs), dtype=torch.uint8, device="cuda")
+    centres = None
+    enc_ms = 0.0
+    for start, x, centres in gaussian_mixture(torch, n, A8_DIM, 1024, 0.3, 2):
+        r = x.shape[0]
+        enc_ms += event_time(torch, stream, lambda: sp.encode(x, r, codes[start:start + r], stream=sh), 1)
+    _, q, _ = next(gaussian_mixture(torch, m, A8_DIM, 1024, 0.3, 3, centres=centres))
+    prep = torch.empty((m, sp.prep_stride), dtype=torch.float32, device="cuda")
+    out = torch.empty((m, n), dtype=torch.float32, device="cuda")
+
+    def run(mode):
+        sp.prepare(q, m, prep, stream=sh)
+        sp.scan(prep, m, codes, n, out, mode=mode, stream=sh)
+
+    run(capi.TQ_MODE_MFMA)
+    ms_m = event_time(torch, stream, lambda: run(capi.TQ_MODE_MFMA), 3)
+    ms_f = event_time(torch, stream, lambda: run(capi.TQ_MODE_FP32), 1)
+    res = {"dim": A8_DIM, "bits": A8_BITS, "codes": n, "queries": m,
+           "encode_vec_per_s": n / (enc_ms / 1e3),
+           "mfma_ms": ms_m, "mfma_dist_per_s": m * n / (ms_m / 1e3),
+           "fp32_ms": ms_f, "fp32_dist_per_s": m * n / (ms_f / 1e3), "mfma_over_fp32": ms_f / ms_m}
+    log(f"asym 8-bit {m} x {n}: mfma {ms_m:.2f} ms ({res['mfma_dist_per_s'] / 1e9:.1f} G dist/s), "
+        f"fp32 {ms_f:.1f} ms ({res['fp32_dist_per_s'] / 1e9:.2f} G dist/s); "
+        f"encode {res['encode_vec_per_s'] / 1e6:.1f} M vec/s")
+    del out, codes, prep
+    torch.cuda.empty_cache()
+    return res
+
+
 def bench_recall(torch, capi, sh, n=RECALL_N, m=RECALL_Q, k=10):
     """Recall@1 and recall@10 of the quantized searches against the exact
     neighbours (squared L2 over the raw vectors, fp64 brute force on the GPU)
@@ -419,6 +460,8 @@ def main():
     ap.add_argument("--no-recall", action="store_true", help="skip the recall measurement")
     ap.add_argument("--no-symmetric", action="store_true",
                     help="skip the symmetric (dim 1000, 4,096 x 2M) measurement")
+    ap.add_argument("--no-asym8", action="store_true",
+                    help="skip the 8-bit (dim 768, 1,024 x 1M) measurement")
     ap.add_argument("--serve-seconds", type=float, default=12.0)
     ap.add_argument("--qps-mfma", type=float, default=40.0)
     ap.add_argument("--qps-fp32", type=float, default=4.0)
@@ -534,6 +577,7 @@ def main():
     torch.cuda.empty_cache()
     recall = None if args.no_recall else bench_recall(torch, capi, sh)
     symmetric = None if args.no_symmetric else bench_symmetric(torch, capi, stream, sh)
+    asym8 = None if args.no_asym8 else bench_asym8(torch, capi, stream, sh)
     batched = None if args.no_batched else bench_batched(torch, capi, stream, sh)
     serving = None
     if not args.no_serving:
@@ -557,6 +601,7 @@ def main():
         "encode_vec_per_s": enc_vps,
         "recall": recall,
         "symmetric": symmetric,
+        "asym_8bit": asym8,
         "batched_uneven": batched,
         "serving": serving,
     }
