@@ -96,9 +96,10 @@ def uneven_batches(seed=15, count=BATCH_COUNT, mmax=512):
 
 def bench_batched(torch, capi, stream, sh, reps=5):
     """256 query batches of uneven m against 5 collections of 500,000 codes:
-    one persistent search plan (MFMA, fused top-10, tuned, HIP-graph replay)
-    against the same batches as per-batch plain launches (prepare, fp32
-    distance_m_to_n, row top-10) and per-batch MFMA launches."""
+    one tuned MFMA search plan (top-10, HIP-graph replay) and the same plan
+    with the fused persistent form forced, against the same batches as
+    per-batch plain launches (prepare, fp32 distance_m_to_n, row top-10) and
+    per-batch MFMA launches."""
     colls = []
     for i, (dim, bits) in enumerate(BATCH_COLLS):
         sp = capi.Space(dim, bits)
@@ -124,20 +125,33 @@ def bench_batched(torch, capi, stream, sh, reps=5):
 
     out = {"batches": len(batches), "queries": total_q, "codes_per_collection": BATCH_N,
            "collections": [{"dim": d, "bits": b} for d, b in BATCH_COLLS], "k": 10}
+    # The tuned plan: per collection the autotuner's fastest form (blocked or
+    # fused) and fused knobs; then the same batches with the fused form forced
+    # (one persistent launch over every collection, tuned knobs).
     plan = capi.SearchPlan(jobs, 10, capi.TQ_MODE_MFMA)
     t0 = time.perf_counter()
-    qt, ipw = plan.tune(stream=sh)
+    qt, ipw, form = plan.tune(stream=sh)
     out["tune_s"] = time.perf_counter() - t0
-    items, wgs = plan.stats()
     plan.run_graph(stream=sh)
     ms = event_time(torch, stream, lambda: plan.run_graph(stream=sh), reps)
-    out["persistent_mfma"] = {"ms": ms, "dist_per_s": dists / (ms / 1e3),
+    forms = {capi.TQ_FORM_FUSED: "fused", capi.TQ_FORM_BLOCKED: "blocked"}
+    out["tuned_mfma_plan"] = {"ms": ms, "dist_per_s": dists / (ms / 1e3),
                               "queries_per_s": total_q / (ms / 1e3),
-                              "tflops": flops / (ms / 1e3) / 1e12, "work_items": items,
-                              "workgroups": wgs, "tuned": {"query_tile": qt, "items_per_wg": ipw}}
+                              "tflops": flops / (ms / 1e3) / 1e12,
+                              "form_of_costliest_collection": forms.get(form, form),
+                              "tuned_fused_knobs": {"query_tile": qt, "items_per_wg": ipw}}
     ref_ids = [j[6].clone() for j in jobs]
     plan.close()
-    log(f"batched: persistent MFMA graph {ms:.2f} ms ({dists / (ms / 1e3) / 1e9:.1f} G dist/s)")
+    log(f"batched: tuned MFMA plan graph {ms:.2f} ms ({dists / (ms / 1e3) / 1e9:.1f} G dist/s)")
+    fplan = capi.SearchPlan(jobs, 10, capi.TQ_MODE_MFMA, form=capi.TQ_FORM_FUSED)
+    items, wgs = fplan.stats()
+    fplan.run_graph(stream=sh)
+    msf = event_time(torch, stream, lambda: fplan.run_graph(stream=sh), reps)
+    out["fused_persistent_mfma"] = {"ms": msf, "dist_per_s": dists / (msf / 1e3),
+                                    "work_items": items, "workgroups": wgs}
+    out["fused_ids_equal_tuned"] = all(torch.equal(a, j[6]) for a, j in zip(ref_ids, jobs))
+    fplan.close()
+    log(f"batched: fused persistent MFMA graph {msf:.2f} ms")
 
     plan32 = capi.SearchPlan(jobs, 10, capi.TQ_MODE_FP32)
     plan32.run_graph(stream=sh)
@@ -165,8 +179,8 @@ def bench_batched(torch, capi, stream, sh, reps=5):
         out[name] = {"ms": msb, "dist_per_s": dists / (msb / 1e3)}
         log(f"batched: {name} {msb:.2f} ms")
     torch.cuda.synchronize()
-    # The MFMA full matrices give the same top-10 ids as the fused search.
-    out["per_batch_mfma_ids_equal_fused"] = all(torch.equal(a, j[6]) for a, j in zip(ref_ids, jobs))
+    # The MFMA full matrices give the same top-10 ids as the tuned plan.
+    out["per_batch_mfma_ids_equal_plan"] = all(torch.equal(a, j[6]) for a, j in zip(ref_ids, jobs))
     out["speedup_vs_per_batch_fp32"] = out["per_batch_fp32"]["ms"] / ms
     del dist, prep, colls, jobs, queries
     torch.cuda.empty_cache()

@@ -191,25 +191,34 @@ typedef struct tq_search_job {
   int64_t *out_ids;     /* [m, k] row index into codes; -1 (dist +inf) past n */
 } tq_search_job;
 
-/* Knobs of the persistent launch (TQ_MODE_MFMA plans). 0 = automatic: the
- * autotuner's table for the job's (padded dim, bits, m) bucket if it has an
- * entry (tq_search_plan_tune), else the built-in rule. */
+/* Forms of an MFMA search over one collection. */
+#define TQ_FORM_AUTO 0
+#define TQ_FORM_FUSED 1   /* the persistent launch with the top-k fused into the scan */
+#define TQ_FORM_BLOCKED 2 /* MFMA distance blocks over chunks of codes, row top-k, fold */
+
+/* Knobs of TQ_MODE_MFMA plans. 0 = automatic: the autotuner's table for the
+ * job's (padded dim, bits, m) bucket if it has an entry
+ * (tq_search_plan_tune), else the built-in rule (blocked; fused where a batch
+ * has more than 65535 queries or no codes). query_tile and items_per_wg
+ * shape the fused form's work queue. */
 typedef struct tq_search_tuning {
   int32_t query_tile;   /* 32 or 64 queries per work item, or 0 */
   int32_t items_per_wg; /* work items per persistent workgroup (load balance), or 0 */
+  int32_t form;         /* TQ_FORM_*, or 0 */
 } tq_search_tuning;
 
 typedef struct tq_search_plan_s *tq_search_plan;
 
 /* Builds a reusable search over `jobs` (copied): checks every job, sizes the
- * work queue and allocates all device scratch once. mode TQ_MODE_MFMA: one
- * persistent launch over a (query tile, code range) queue with the top-k
- * fused into the scan (k <= TQ_SEARCH_MAX_K); queries aimed at a collection
- * of 2,000,000 codes or more run blocked instead (MFMA distances of a chunk
- * of codes into a scratch block of at most 256 MB, a row top-k per chunk,
- * candidates folded per query), with the same results. TQ_MODE_FP32: per
- * job the plain fp32 distance matrix in row blocks, then a row top-k.
- * `tuning` may be NULL. */
+ * work queue and allocates all device scratch once. Jobs aimed at one
+ * collection are scored as one batch. mode TQ_MODE_MFMA, per batch in the
+ * form the knobs give (k <= TQ_SEARCH_MAX_K): blocked (MFMA distances of
+ * the batch against a chunk of codes into a scratch block of at most 256 MB
+ * or 4,096 columns, a row top-k per chunk, candidates folded per query), or
+ * fused (every fused batch of the plan in one persistent launch over a
+ * (query tile, code range) queue, the top-k lists kept in registers); both
+ * give the same results. TQ_MODE_FP32: per job the plain fp32 distance
+ * matrix in row blocks, then a row top-k. `tuning` may be NULL. */
 int tq_search_plan_create(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
                           const tq_search_tuning *tuning, tq_search_plan *out);
 /* Enqueues the plan on `stream`: query preparation, the scan and the top-k.
@@ -220,9 +229,12 @@ int tq_search_plan_run(tq_search_plan p, void *stream);
  * on the first call). */
 int tq_search_plan_run_graph(tq_search_plan p, void *stream);
 /* Times the candidate knobs for each (padded dim, bits, m) bucket of the
- * plan's jobs on the plan's own buffers (overwriting the outputs), records
- * the fastest in the process-wide table, and rebuilds the plan with them.
- * Synchronous. `chosen` (may be NULL) receives the plan's resulting knobs. */
+ * plan's jobs on the plan's own buffers (overwriting the outputs): the
+ * blocked form and the fused form with query_tile in {32, 64} x
+ * items_per_wg in {1, 2, 4, 8}. Records the fastest form and the fastest
+ * fused knobs in the process-wide table, and rebuilds the plan with them.
+ * Synchronous. `chosen` (may be NULL) receives the plan's resulting knobs
+ * (query_tile and form of its costliest batch). */
 int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen);
 /* Work items and persistent workgroups of the plan (either may be NULL). */
 int tq_search_plan_stats(tq_search_plan p, uint64_t *items, uint64_t *workgroups);

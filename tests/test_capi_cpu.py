@@ -73,6 +73,10 @@ def test_search_entry_points_validate_before_device_use():
     bad = capi.SearchTuning(48, 0)
     assert L.tq_search_plan_create(jobs, 1, 10, capi.TQ_MODE_MFMA, ctypes.byref(bad),
                                    ctypes.byref(h)) == capi.TQ_EINVAL
+    bad_form = capi.SearchTuning(0, 0, 3)
+    assert L.tq_search_plan_create(jobs, 1, 10, capi.TQ_MODE_MFMA, ctypes.byref(bad_form),
+                                   ctypes.byref(h)) == capi.TQ_EINVAL
+    assert b"form must be" in L.tq_last_error()
     assert L.tq_topk_rows(None, 1, 1, 17, None, None, None) == capi.TQ_EINVAL
     assert L.tq_topk_rows(None, 70000, 1, 10, None, None, None) == capi.TQ_EINVAL
     assert L.tq_search_plan_run(None, None) == capi.TQ_EINVAL

@@ -1,14 +1,13 @@
-"""MFMA search over collections of at least 2,000,000 codes, which plans run
-blocked (MFMA distance blocks over chunks of codes, row top-k per chunk,
-folded into one list per query) instead of through the fused persistent
-kernel.
+"""MFMA search in the blocked form (MFMA distance blocks over chunks of codes,
+row top-k per chunk, folded into one list per query; the plans' default) over
+collections long enough to take several chunks.
 
 Bars: the plan's top-k equals the (distance, id)-ordered top-k of the full
 MFMA distance matrix, distances bit for bit, with one chunk and with several
 (m = 5, 37, 100 over 2.3M codes: 1, 2 and 4 chunks), for 4-bit and 8-bit
 codes; a plan holding a blocked and a fused batch gives each job what it gets
-alone; graph replays equal eager runs; the server over such a collection
-returns what a one-shot plan returns.
+alone (blocked); graph replays equal eager runs; the server over such a
+collection returns what a one-shot plan returns.
 """
 import sys
 
@@ -119,19 +118,25 @@ def test_blocked_byte_codes(gpu, m):
 
 
 def test_mixed_blocked_and_fused(gpu, big):
+    # Two jobs over a second collection coalesce into one batch of 70,000
+    # queries, more than the blocked form takes, so that batch runs fused in
+    # the same plan as the blocked batches over the long collection; alone,
+    # each of those jobs runs blocked. Every job gets the same bits either way.
     import torch
     capi = gpu
     sp, codes, g = big
     say("[gpu] blocked + fused batches in one plan")
     sp2 = capi.Space(96, 4)
-    n2 = 300_000
+    n2 = 60_000
     x2 = torch.randn((n2, 96), generator=g, device="cuda")
     codes2 = torch.empty((n2, sp2.cs), dtype=torch.uint8, device="cuda")
     sp2.encode(x2, n2, codes2)
     Q1 = torch.randn((20, 128), generator=g, device="cuda")
-    Q2 = torch.randn((33, 96), generator=g, device="cuda")
+    Q2 = torch.randn((40_000, 96), generator=g, device="cuda")
     Q3 = torch.randn((7, 128), generator=g, device="cuda")
-    jobs = [(sp, Q1, 20, codes, N_BIG), (sp2, Q2, 33, codes2, n2), (sp, Q3, 7, codes, N_BIG)]
+    Q4 = torch.randn((30_000, 96), generator=g, device="cuda")
+    jobs = [(sp, Q1, 20, codes, N_BIG), (sp2, Q2, 40_000, codes2, n2), (sp, Q3, 7, codes, N_BIG),
+            (sp2, Q4, 30_000, codes2, n2)]
     mixed = plan_topk(torch, capi, jobs, K)
     for job, (md, mi) in zip(jobs, mixed):
         (ad, ai), = plan_topk(torch, capi, [job], K)

@@ -93,18 +93,20 @@ CASES = [(128, 4), (96, 4), (768, 4), (1000, 4), (128, 8), (384, 8), (1536, 4)]
 
 
 @pytest.mark.parametrize("dim,bits", CASES)
-@pytest.mark.parametrize("qt", [32, 64])
-def test_fused_topk_equals_topk_of_mfma_matrix(gpu, torch_cuda, dim, bits, qt):
+@pytest.mark.parametrize("form,qt", [(1, 32), (1, 64), (2, 0)])
+def test_mfma_search_equals_topk_of_mfma_matrix(gpu, torch_cuda, dim, bits, form, qt):
+    # form 1: the fused persistent launch with 32- or 64-query tiles;
+    # form 2: blocked (distance blocks + row top-k)
     torch, capi = torch_cuda, gpu
     n = 3000 if dim < 1000 else 1300
-    say(f"[gpu] fused top-k dim={dim} bits={bits} qt={qt} n={n}")
+    say(f"[gpu] MFMA search form={form} dim={dim} bits={bits} qt={qt} n={n}")
     c = Coll(torch, capi, dim, bits, n, seed=dim + bits, dup=200)
     for m in (1, 37, 70):
         Q, dQ = c.queries(torch, m)
         k = 10
         od, oi = outputs(torch, m, k)
         plan = capi.SearchPlan([(c.sp, dQ, m, c.codes, n, od, oi)], k, capi.TQ_MODE_MFMA,
-                               query_tile=qt, items_per_wg=2)
+                               query_tile=qt, items_per_wg=2 if qt else 0, form=form)
         plan.run()
         torch.cuda.synchronize()
         want_d, want_i = host_topk(full_matrix(torch, capi, c, dQ, m, capi.TQ_MODE_MFMA), k)
@@ -159,17 +161,20 @@ def test_mixed_plan_equals_each_job_alone(gpu, torch_cuda):
         od, oi = outputs(torch, m, k)
         jobs.append((c.sp, dQ, m, c.codes, c.n, od, oi))
         qs.append((ci, m, dQ))
-    for mode in (capi.TQ_MODE_MFMA, capi.TQ_MODE_FP32):
-        plan = capi.SearchPlan(jobs, k, mode)
+    forms = {}
+    for mode, form in ((capi.TQ_MODE_MFMA, capi.TQ_FORM_AUTO), (capi.TQ_MODE_MFMA, capi.TQ_FORM_FUSED),
+                       (capi.TQ_MODE_FP32, capi.TQ_FORM_AUTO)):
+        plan = capi.SearchPlan(jobs, k, mode, form=form)
         items, wgs = plan.stats()
-        say(f"  mode={mode} items={items} workgroups={wgs}")
+        say(f"  mode={mode} form={form} items={items} workgroups={wgs}")
         plan.run()
         torch.cuda.synchronize()
         mixed = [(j[5][: j[2]].cpu().numpy(), j[6][: j[2]].cpu().numpy()) for j in jobs]
+        forms[(mode, form)] = mixed
         for (ci, m, dQ), (md, mi) in zip(qs, mixed):
             c = colls[ci]
             od, oi = outputs(torch, m, k)
-            p1 = capi.SearchPlan([(c.sp, dQ, m, c.codes, c.n, od, oi)], k, mode)
+            p1 = capi.SearchPlan([(c.sp, dQ, m, c.codes, c.n, od, oi)], k, mode, form=form)
             p1.run()
             torch.cuda.synchronize()
             np.testing.assert_array_equal(oi[:m].cpu().numpy(), mi)
@@ -180,6 +185,11 @@ def test_mixed_plan_equals_each_job_alone(gpu, torch_cuda):
                 assert (mi == -1).all() and np.isinf(md).all()
             p1.close()
         plan.close()
+    # The blocked and the fused form give the same bits.
+    for (ad, ai), (fd, fi) in zip(forms[(capi.TQ_MODE_MFMA, capi.TQ_FORM_AUTO)],
+                                  forms[(capi.TQ_MODE_MFMA, capi.TQ_FORM_FUSED)]):
+        np.testing.assert_array_equal(ai, fi)
+        np.testing.assert_array_equal(ad.view(np.uint32), fd.view(np.uint32))
 
 
 def test_graph_replays_read_current_queries(gpu, torch_cuda):
@@ -192,8 +202,9 @@ def test_graph_replays_read_current_queries(gpu, torch_cuda):
     _, q2 = c2.queries(torch, 7)
     o1, o2 = outputs(torch, 50, k), outputs(torch, 7, k)
     jobs = [(c1.sp, q1, 50, c1.codes, 4000, *o1), (c2.sp, q2, 7, c2.codes, 3000, *o2)]
-    for mode in (capi.TQ_MODE_MFMA, capi.TQ_MODE_FP32):
-        plan = capi.SearchPlan(jobs, k, mode)
+    for mode, form in ((capi.TQ_MODE_MFMA, capi.TQ_FORM_AUTO), (capi.TQ_MODE_MFMA, capi.TQ_FORM_FUSED),
+                       (capi.TQ_MODE_FP32, capi.TQ_FORM_AUTO)):
+        plan = capi.SearchPlan(jobs, k, mode, form=form)
         for rep in range(3):
             # New queries written in place between replays, with eager work
             # and a synchronisation in between.
@@ -208,7 +219,7 @@ def test_graph_replays_read_current_queries(gpu, torch_cuda):
             plan.run()
             torch.cuda.synchronize()
             for a, b in zip(g, (*o1, *o2)):
-                assert torch.equal(a, b), (mode, rep)
+                assert torch.equal(a, b), (mode, form, rep)
             want_d, want_i = host_topk(full_matrix(torch, capi, c1, q1, 50, mode), k)
             np.testing.assert_array_equal(o1[1].cpu().numpy(), want_i)
         plan.close()
@@ -224,13 +235,23 @@ def test_tuned_plan_gives_the_same_results(gpu, torch_cuda):
     plan.run()
     torch.cuda.synchronize()
     before = (od.clone(), oi.clone())
-    qt, ipw = plan.tune()
-    assert qt in (32, 64) and ipw in (1, 2, 4, 8)
+    qt, ipw, form = plan.tune()
+    assert qt in (32, 64) and ipw in (1, 2, 4, 8) and form in (capi.TQ_FORM_FUSED, capi.TQ_FORM_BLOCKED)
     od.fill_(0)
     plan.run()
     torch.cuda.synchronize()
     assert torch.equal(before[0], od) and torch.equal(before[1], oi)
     plan.close()
+    # A plan that forces the fused form takes the tuned fused knobs and gives
+    # the same results.
+    od.fill_(0)
+    fused = capi.SearchPlan([(c.sp, dQ, 20, c.codes, c.n, od, oi)], 10, capi.TQ_MODE_MFMA,
+                            form=capi.TQ_FORM_FUSED)
+    fused.run()
+    torch.cuda.synchronize()
+    assert torch.equal(before[0], od) and torch.equal(before[1], oi)
+    assert fused.stats()[0] > 0
+    fused.close()
 
 
 def test_k_above_n_and_one_shot_search(gpu, torch_cuda):
@@ -249,12 +270,25 @@ def test_k_above_n_and_one_shot_search(gpu, torch_cuda):
         np.testing.assert_array_equal(got_i, want_i)
 
 
-def test_topk_rows_ties_by_column(gpu, torch_cuda):
+@pytest.mark.parametrize("kind", ["ties", "uniform", "descending"])
+def test_topk_rows_ties_by_column(gpu, torch_cuda, kind):
+    # Segments of 8,192 columns go to the wavefronts as (row, segment) pairs,
+    # so rows of 1-3 segments share workgroups; the shapes cover a lone
+    # column in the last segment, partial sub-blocks of 2,048 columns, rows
+    # shorter than one sub-block, and (ties, descending rows) more entries at
+    # or below the first sub-block's bound than the packed list holds.
     torch, capi = torch_cuda, gpu
-    say("[gpu] topk rows")
+    say(f"[gpu] topk rows {kind}")
     rng = np.random.default_rng(5)
-    for m, n, k in ((3, 100, 10), (20, 70000, 16), (1, 1, 1), (5, 4097, 7)):
-        D = rng.integers(0, 50, (m, n)).astype(np.float32)  # many exact ties
+    for m, n, k in ((3, 100, 10), (20, 70000, 16), (1, 1, 1), (5, 4097, 7), (7, 8193, 16),
+                    (9, 13568, 10), (33, 2047, 16), (6, 2049, 3), (4, 24577, 16)):
+        if kind == "ties":
+            D = rng.integers(0, 50, (m, n)).astype(np.float32)  # many exact ties
+        elif kind == "uniform":
+            D = rng.random((m, n), dtype=np.float32)
+        else:
+            D = np.tile(np.arange(n, 0, -1, dtype=np.float32), (m, 1))
+            D += rng.integers(0, 2, (m, 1)).astype(np.float32)
         dD = torch.from_numpy(D).cuda()
         od = torch.empty((m, k), dtype=torch.float32, device="cuda")
         oi = torch.empty((m, k), dtype=torch.int64, device="cuda")

@@ -38,9 +38,11 @@ int plan_new(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
 size_t plan_ws_bytes(tq_search_plan p);
 int plan_bind(tq_search_plan p, void *ws, size_t bytes, hipStream_t st);
 int plan_enqueue(tq_search_plan p, hipStream_t st);
-// MFMA plans run a batch of `rows` queries over `n` codes blocked (distance
-// blocks + row top-k) when plan_blocked(n, rows); plan_blocked_ws_bound(rows)
-// bounds the workspace that adds for batches of up to `rows` queries.
+// MFMA plans can run a batch of `rows` queries over `n` codes blocked
+// (distance blocks + row top-k) when plan_blocked(n, rows), and do unless
+// their knobs ask for the fused form; plan_blocked_ws_bound(rows, n_max)
+// bounds the workspace that adds for batches of up to `rows` queries over at
+// most n_max codes.
 bool plan_blocked(long n, long rows);
-size_t plan_blocked_ws_bound(long rows);
+size_t plan_blocked_ws_bound(long rows, long n_max);
 }  // namespace tq

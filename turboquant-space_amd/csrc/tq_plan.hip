@@ -85,16 +85,17 @@ void tune_store(int D, int bits, long m, const tq_search_tuning &t) {
 
 constexpr int kDefaultItemsPerWg = 4;
 constexpr size_t kFp32BlockBytes = 512ull << 20;  // distance rows per block (FP32 plans)
-// MFMA plans: a batch over a collection of at least kBlockedMinCodes codes
-// runs blocked: k_mfma writes the distances of all its queries against a
-// chunk of codes into a block of at most kMfmaBlockBytes, a row top-k keeps
-// each chunk's candidates, and the candidates fold into one list per query.
-// The fused persistent search (k_search) keeps its top-k lists in registers,
-// which limits its query tiles to 32 rows at two wavefronts per SIMD (or 64
-// rows at one); on long collections the blocked form is 1.3-3x faster at
-// every batch size measured (m = 1..256, 10M codes), on short ones with
-// many small batches the single persistent launch wins.
-constexpr long kBlockedMinCodes = 2000000;
+// MFMA plans: by default a batch runs blocked: k_mfma writes the distances
+// of all its queries against a chunk of codes into a block of at most
+// kMfmaBlockBytes (or 4,096 columns), a row top-k keeps each chunk's
+// candidates, and the candidates fold into one list per query. The fused
+// persistent search (k_search) keeps its top-k lists in registers, which
+// limits its query tiles to 32 rows at two wavefronts per SIMD (or 64 rows at
+// one); the blocked form was faster at every shape measured (one job, dim
+// 768 4-bit and dim 128 8-bit, n = 50,000 .. 2,000,000, m = 1 .. 1,024:
+// 0.24-0.93x the fused time; 10M codes: 1.3-3x). The fused form stays for
+// batches the blocked one cannot take (more than 65535 queries, no codes),
+// and for callers and the autotuner to choose.
 constexpr size_t kMfmaBlockBytes = 256ull << 20;
 
 size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
@@ -216,6 +217,7 @@ tq_search_tuning knobs_for(const tq_search_plan_s *p, const PlanBatch &b) {
   t.items_per_wg = p->forced.items_per_wg ? p->forced.items_per_wg
                    : have                 ? tab.items_per_wg
                                           : tq::kDefaultItemsPerWg;
+  t.form = p->forced.form ? p->forced.form : have && tab.form ? tab.form : TQ_FORM_BLOCKED;
   return t;
 }
 
@@ -240,7 +242,8 @@ int layout(tq_search_plan_s *p) {
   for (PlanBatch &b : p->batches) {
     const tq_search_tuning t = knobs_for(p, b);
     b.qr = t.query_tile == 64 ? 2 : 1;
-    b.blocked = p->mode == TQ_MODE_MFMA && b.space->info.padded_dim >= 32 && tq::plan_blocked(b.n, b.m);
+    b.blocked = p->mode == TQ_MODE_MFMA && t.form == TQ_FORM_BLOCKED &&
+                b.space->info.padded_dim >= 32 && tq::plan_blocked(b.n, b.m);
     const double cost = (double)b.m * (double)b.n * (double)b.space->info.padded_dim;
     if (cost > worst) {
       worst = cost;
@@ -561,6 +564,9 @@ int plan_new(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
       return fail(TQ_EINVAL, "%s: query_tile must be 0, 32 or 64", what);
     if (tuning->items_per_wg < 0 || tuning->items_per_wg > 1024)
       return fail(TQ_EINVAL, "%s: items_per_wg must be in [0, 1024]", what);
+    if (tuning->form != TQ_FORM_AUTO && tuning->form != TQ_FORM_FUSED &&
+        tuning->form != TQ_FORM_BLOCKED)
+      return fail(TQ_EINVAL, "%s: form must be 0, TQ_FORM_FUSED or TQ_FORM_BLOCKED", what);
   }
   int device = -1;
   for (uint64_t i = 0; i < njobs; ++i) {
@@ -633,15 +639,19 @@ int plan_bind(tq_search_plan p, void *ws, size_t bytes, hipStream_t st) {
 int plan_enqueue(tq_search_plan p, hipStream_t st) { return enqueue(p, st); }
 
 bool plan_blocked(long n, long rows) {
-  return rows > 0 && rows <= 65535 && n >= kBlockedMinCodes && n < 0xFFFFFFFFL;
+  return rows > 0 && rows <= 65535 && n > 0 && n < 0xFFFFFFFFL;
 }
 
-size_t plan_blocked_ws_bound(long rows) {
-  // The distance block, and (1 + segments) lists per query over a chunk of
-  // nc <= max(4096, kMfmaBlockBytes / (4 * rows)) codes, segments <=
-  // ceil(nc / 2048): rows * lists <= 4 * rows + kMfmaBlockBytes / 8192.
-  const size_t lists = 4 * (size_t)rows + kMfmaBlockBytes / 8192;
-  return align_up(kMfmaBlockBytes) + 2 * align_up(lists * kSearchMaxK * sizeof(float)) + 4096;
+size_t plan_blocked_ws_bound(long rows, long n_max) {
+  // The distance block: rows x nc floats, nc <= min(n, max(4096,
+  // kMfmaBlockBytes / (4 * rows))); and (1 + segments) lists per query over a
+  // chunk, segments <= ceil(nc / 2048): rows * lists <= 4 * rows +
+  // kMfmaBlockBytes / 8192.
+  const size_t r = (size_t)std::max(rows, 1L);
+  const size_t block = std::min(std::max(kMfmaBlockBytes, r * 4096 * sizeof(float)),
+                                r * (size_t)std::max(n_max, 1L) * sizeof(float));
+  const size_t lists = 4 * r + kMfmaBlockBytes / 8192;
+  return align_up(block) + 2 * align_up(lists * kSearchMaxK * sizeof(float)) + 4096;
 }
 
 }  // namespace tq
@@ -719,36 +729,45 @@ int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen
       if (b.m == 0 || b.n == 0) continue;
       std::vector<tq_search_job> sub;
       for (int ji : b.jobs) sub.push_back(p->jobs[ji]);
+      // Candidates: the fused form's knobs, then the blocked form (when the
+      // batch allows it). The table keeps the fastest fused knobs and the
+      // fastest form.
       tq_search_tuning best{};
-      float best_ms = 0.0f;
-      const int tiles[2] = {32, 64};
-      const int ipws[4] = {1, 2, 4, 8};
-      for (int qt : tiles) {
-        for (int ipw : ipws) {
-          tq_search_tuning cand{qt, ipw};
-          tq_search_plan sp = nullptr;
-          rc = tq_search_plan_create(sub.data(), sub.size(), p->k, p->mode, &cand, &sp);
-          if (rc) break;
-          float ms = 0.0f, run_best = 0.0f;
-          for (int rep = 0; rep < 4 && rc == TQ_OK; ++rep) {
-            if ((e = hipEventRecord(t0, st)) != hipSuccess) rc = hip_fail(e, what);
-            if (!rc) rc = enqueue(sp, st);
-            if (!rc && (e = hipEventRecord(t1, st)) != hipSuccess) rc = hip_fail(e, what);
-            if (!rc && (e = hipEventSynchronize(t1)) != hipSuccess) rc = hip_fail(e, what);
-            if (!rc && (e = hipEventElapsedTime(&ms, t0, t1)) != hipSuccess) rc = hip_fail(e, what);
-            if (!rc && rep > 0 && (rep == 1 || ms < run_best)) run_best = ms;  // rep 0 warms up
-          }
-          tq_search_plan_destroy(sp);
-          if (rc) break;
-          if (best.query_tile == 0 || run_best < best_ms) {
-            best = cand;
-            best_ms = run_best;
-          }
-        }
+      float best_ms = 0.0f, best_form_ms = 0.0f;
+      int best_form = TQ_FORM_FUSED;
+      std::vector<tq_search_tuning> cands;
+      for (int qt : {32, 64})
+        for (int ipw : {1, 2, 4, 8}) cands.push_back(tq_search_tuning{qt, ipw, TQ_FORM_FUSED});
+      if (b.space->info.padded_dim >= 32 && tq::plan_blocked(b.n, b.m))
+        cands.push_back(tq_search_tuning{0, 0, TQ_FORM_BLOCKED});
+      for (const tq_search_tuning &cand : cands) {
+        tq_search_plan sp = nullptr;
+        rc = tq_search_plan_create(sub.data(), sub.size(), p->k, p->mode, &cand, &sp);
         if (rc) break;
+        float ms = 0.0f, run_best = 0.0f;
+        for (int rep = 0; rep < 4 && rc == TQ_OK; ++rep) {
+          if ((e = hipEventRecord(t0, st)) != hipSuccess) rc = hip_fail(e, what);
+          if (!rc) rc = enqueue(sp, st);
+          if (!rc && (e = hipEventRecord(t1, st)) != hipSuccess) rc = hip_fail(e, what);
+          if (!rc && (e = hipEventSynchronize(t1)) != hipSuccess) rc = hip_fail(e, what);
+          if (!rc && (e = hipEventElapsedTime(&ms, t0, t1)) != hipSuccess) rc = hip_fail(e, what);
+          if (!rc && rep > 0 && (rep == 1 || ms < run_best)) run_best = ms;  // rep 0 warms up
+        }
+        tq_search_plan_destroy(sp);
+        if (rc) break;
+        if (cand.form == TQ_FORM_FUSED && (best.query_tile == 0 || run_best < best_ms)) {
+          best = cand;
+          best_ms = run_best;
+        }
+        if (&cand == &cands[0] || run_best < best_form_ms) {
+          best_form = cand.form;
+          best_form_ms = run_best;
+        }
       }
-      if (rc == TQ_OK)
+      if (rc == TQ_OK) {
+        best.form = best_form;
         tq::tune_store((int)b.space->info.padded_dim, b.space->info.bits_per_coord, b.m, best);
+      }
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
@@ -758,13 +777,14 @@ int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen
   }
   if (chosen) {
     *chosen = p->used;
-    // The query tile of the costliest batch.
+    // The query tile and form of the costliest batch.
     double worst = -1.0;
     for (const PlanBatch &b : p->batches) {
       const double cost = (double)b.m * (double)b.n * (double)b.space->info.padded_dim;
       if (cost > worst) {
         worst = cost;
         chosen->query_tile = 32 * b.qr;
+        chosen->form = b.blocked ? TQ_FORM_BLOCKED : TQ_FORM_FUSED;
       }
     }
   }

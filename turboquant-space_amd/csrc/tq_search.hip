@@ -491,11 +491,13 @@ __device__ __forceinline__ void wave_pop_list(float (&td)[L], uint32_t (&ti)[L],
   }
 }
 
-// Row-wise top-k of a dense distance matrix, stage 1: wavefront w of
-// workgroup (x, row) owns segments seg = 4x + w (+ 4 * gridDim.x ...) of
-// kTopkSeg columns, read as sub-blocks of kTopkSub columns (32 per lane, in
-// registers), and writes each segment's best K as list seg0 + seg of the
-// row's `stride` lists (ids = column + id0).
+// Row-wise top-k of a dense distance matrix, stage 1: the m x nseg
+// (row, segment) pairs of kTopkSeg columns, row-major, go to the wavefronts
+// in turn (wavefront w of workgroup x takes pairs 4x + w, + 4 * gridDim.x,
+// ...; rows shorter than four segments share a workgroup), each segment read
+// as sub-blocks of kTopkSub columns (32 per lane, in registers); a segment's
+// best K go to list seg0 + seg of its row's `stride` lists (ids = column +
+// id0).
 //   1. Every lane's minimum over the first sub-block; thr = the K-th smallest
 //      of the 64 minima. They are K distinct entries of the segment, so the
 //      segment's K best all lie at or below thr.
@@ -510,10 +512,11 @@ __device__ __forceinline__ void wave_pop_list(float (&td)[L], uint32_t (&ti)[L],
 // all in flight together. Only the wavefront's own lanes touch its LDS list:
 // no barrier.
 constexpr int kTopkSub = 2048;
+constexpr int kBufferWord3 = 0x00020000;  // raw 32-bit buffer resource (gfx9 DATA_FORMAT)
 constexpr int kTopkSeg = 4 * kTopkSub;
 constexpr int kTopkCap = 256;
 template <int K>
-__global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dist, long n,
+__global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dist, long m, long n,
                                                    long nseg, long stride, long seg0, uint32_t id0,
                                                    float *__restrict__ seg_d,
                                                    uint32_t *__restrict__ seg_i) {
@@ -522,21 +525,26 @@ __global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dis
   __shared__ uint32_t c_i[4][kTopkCap];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  const long row = blockIdx.y;
-  const float *p = dist + row * n;
-  auto load_sub = [&](float (&v)[PL], long cb) __attribute__((always_inline)) {
+  // Within a segment, columns are 32-bit offsets jl < nl from the segment's
+  // first column. The segment is read through a buffer resource of nl * 4
+  // bytes: the hardware returns 0 for offsets past it, so the loads need no
+  // clamping and take a lane offset plus constant offsets (no 64-bit
+  // addresses in registers); columns past nl are then set to +inf.
+  const float *ps = dist;
+  int nl = 0;
+  auto load_sub = [&](float (&v)[PL], __amdgpu_buffer_rsrc_t rs, int sb)
+                      __attribute__((always_inline)) {
 #pragma unroll
-    for (int u = 0; u < PL; ++u) {
-      const long j = cb + u * 64 + lane;
-      v[u] = p[j < n ? j : n - 1];
-    }
+    for (int u = 0; u < PL; ++u)
+      v[u] = __builtin_bit_cast(
+          float, __builtin_amdgcn_raw_buffer_load_b32(rs, 4 * lane, 4 * (sb * kTopkSub + u * 64), 0));
 #pragma unroll
-    for (int u = 0; u < PL; ++u) {
-      const long j = cb + u * 64 + lane;
-      if (j >= n) v[u] = __builtin_inff();
-    }
+    for (int u = 0; u < PL; ++u)
+      if (sb * kTopkSub + u * 64 + lane >= nl) v[u] = __builtin_inff();
   };
-  for (long seg = (long)blockIdx.x * 4 + w; seg < nseg; seg += (long)gridDim.x * 4) {
+  const long pairs = m * nseg;
+  for (long t = (long)blockIdx.x * 4 + w; t < pairs; t += (long)gridDim.x * 4) {
+    const long row = t / nseg, seg = t - row * nseg;
     float *od = seg_d + (row * stride + seg0 + seg) * K;
     uint32_t *oi = seg_i + (row * stride + seg0 + seg) * K;
     if (n <= 0) {  // no columns (dist may be null): an empty list
@@ -547,15 +555,20 @@ __global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dis
       continue;
     }
     const long c0 = seg * kTopkSeg;
+    ps = dist + row * n + c0;
+    nl = (int)(n - c0 < kTopkSeg ? n - c0 : kTopkSeg);  // >= 1
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(ps), 0, 4 * nl, kBufferWord3);
+    const uint32_t idb = (uint32_t)c0 + id0;            // id of column jl: idb + jl
     float v[PL];
-    load_sub(v, c0);
+    load_sub(v, rs, 0);
     // 1. lane minima over the first sub-block, then thr.
     float md = __builtin_inff();
     uint32_t mi = 0xFFFFFFFFu;
 #pragma unroll
     for (int u = 0; u < PL; ++u) {
-      const long j = c0 + u * 64 + lane;
-      const uint32_t id = j < n ? (uint32_t)j + id0 : 0xFFFFFFFFu;
+      const int jl = u * 64 + lane;
+      const uint32_t id = jl < nl ? idb + (uint32_t)jl : 0xFFFFFFFFu;
       if (lex_less(v[u], id, md, mi)) {
         md = v[u];
         mi = id;
@@ -577,14 +590,13 @@ __global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dis
     // 2. pack the entries at or below thr, sub-block by sub-block.
     int cnt = 0;
     for (int sb = 0; sb < kTopkSeg / kTopkSub; ++sb) {
-      const long cb = c0 + (long)sb * kTopkSub;
-      if (cb >= n) break;  // wavefront-uniform
-      if (sb > 0) load_sub(v, cb);
+      if (sb * kTopkSub >= nl) break;  // wavefront-uniform
+      if (sb > 0) load_sub(v, rs, sb);
 #pragma unroll
       for (int u = 0; u < PL; ++u) {
-        const long j = cb + u * 64 + lane;
-        const uint32_t id = (uint32_t)j + id0;
-        const bool cand = j < n && !lex_less(thr_d, thr_i, v[u], id);
+        const int jl = sb * kTopkSub + u * 64 + lane;
+        const uint32_t id = idb + (uint32_t)jl;
+        const bool cand = jl < nl && !lex_less(thr_d, thr_i, v[u], id);
         const uint64_t b = __ballot(cand);
         const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi(
                                   (uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
@@ -612,16 +624,12 @@ __global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dis
       float td[K];
       uint32_t ti[K];
       topk_clear<K>(td, ti);
-      for (int sb = 0; sb < kTopkSeg / kTopkSub; ++sb) {
-        const long cb = c0 + (long)sb * kTopkSub;
-        if (cb >= n) break;
-        load_sub(v, cb);
-#pragma unroll
-        for (int u = 0; u < PL; ++u) {
-          const long j = cb + u * 64 + lane;
-          const uint32_t id = (uint32_t)j + id0;
-          if (j < n && !lex_less(thr_d, thr_i, v[u], id)) topk_insert<K>(td, ti, v[u], id);
-        }
+      // (rare: one column per lane at a time, so v[] is not live here)
+#pragma nounroll
+      for (int jl = lane; jl < nl; jl += 64) {
+        const float d = ps[jl];
+        const uint32_t id = idb + (uint32_t)jl;
+        if (!lex_less(thr_d, thr_i, d, id)) topk_insert<K>(td, ti, d, id);
       }
       wave_pop_list<K, K>(td, ti, od, oi, lane);
     }
@@ -778,6 +786,13 @@ hipError_t launch_search(const SearchJobDev *d_jobs, const SearchItem *d_items,
   return hipGetLastError();
 }
 
+// Workgroups for m x nseg (row, segment) pairs, four per workgroup; past
+// 2^20 workgroups the wavefronts loop.
+unsigned topk_rows_grid(long m, long nseg) {
+  const long wg = (m * nseg + 3) / 4;
+  return (unsigned)(wg < (1L << 20) ? wg : (1L << 20));
+}
+
 long topk_rows_segments(long m, long n) {
   (void)m;
   const long seg = (n + kTopkSeg - 1) / kTopkSeg;  // one wavefront per kTopkSeg columns
@@ -789,9 +804,8 @@ hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out
   if (m <= 0) return hipSuccess;
   if (k < 1 || k > kSearchMaxK || m > 65535 || n >= 0xFFFFFFFFL) return hipErrorInvalidValue;
   const long nseg = topk_rows_segments(m, n);
-  const long gx = (nseg + 3) / 4 < 65535 ? (nseg + 3) / 4 : 65535;
-  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3((unsigned)gx, (unsigned)m), dim3(256), 0, st,
-                     dist, n, nseg, nseg, 0L, 0u, seg_d, seg_i);
+  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3(topk_rows_grid(m, nseg)), dim3(256), 0, st,
+                     dist, m, n, nseg, nseg, 0L, 0u, seg_d, seg_i);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_topk_rows_fold<kSearchMaxK>, dim3((unsigned)((m + 3) / 4)), dim3(256), 0,
@@ -804,9 +818,8 @@ hipError_t launch_topk_rows_chunk(const float *dist, long m, long n, long stride
   if (m <= 0 || n <= 0) return hipSuccess;
   const long nseg = topk_rows_segments(m, n);
   if (m > 65535 || n >= 0xFFFFFFFFL || seg0 + nseg > stride) return hipErrorInvalidValue;
-  const long gx = (nseg + 3) / 4 < 65535 ? (nseg + 3) / 4 : 65535;
-  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3((unsigned)gx, (unsigned)m), dim3(256), 0, st,
-                     dist, n, nseg, stride, seg0, id0, seg_d, seg_i);
+  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3(topk_rows_grid(m, nseg)), dim3(256), 0, st,
+                     dist, m, n, nseg, stride, seg0, id0, seg_d, seg_i);
   return hipGetLastError();
 }
 

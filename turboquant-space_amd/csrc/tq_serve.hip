@@ -16,6 +16,7 @@
 // before any search that counts them.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -230,14 +231,15 @@ size_t batch_bound(tq_server_s *s, const std::vector<Request *> &b) {
   if (any) {
     if (s->cfg.mode == TQ_MODE_MFMA) {
       bytes += (3 * (size_t)s->cus * 8 + qtiles + 16) * 4 * 64 * TQ_SEARCH_MAX_K * 8;
-      long rows = 0;
+      long rows = 0, n_max = 0;
       bool blocked = false;
       for (Request *r : b)
         if (r->kind == kSearch) {
           rows += (long)r->rows;
+          n_max = std::max(n_max, (long)r->n_at);
           blocked = blocked || tq::plan_blocked((long)r->n_at, 1);
         }
-      if (blocked) bytes += tq::plan_blocked_ws_bound(rows);
+      if (blocked) bytes += tq::plan_blocked_ws_bound(rows, n_max);
     } else
       bytes += (512ull << 20) + (8ull << 20);
   }

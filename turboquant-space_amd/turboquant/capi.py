@@ -40,6 +40,7 @@ EXPORTED = [
     "tq_server_stats_get",
 ]
 TQ_SEARCH_MAX_K = 16
+TQ_FORM_AUTO, TQ_FORM_FUSED, TQ_FORM_BLOCKED = 0, 1, 2
 
 
 class SpaceInfo(ctypes.Structure):
@@ -70,7 +71,8 @@ class SearchJob(ctypes.Structure):
 
 
 class SearchTuning(ctypes.Structure):
-    _fields_ = [("query_tile", ctypes.c_int32), ("items_per_wg", ctypes.c_int32)]
+    _fields_ = [("query_tile", ctypes.c_int32), ("items_per_wg", ctypes.c_int32),
+                ("form", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -302,10 +304,11 @@ class EncodeGraph:
 class SearchPlan:
     """Owns a tq_search_plan over device-resident job buffers."""
 
-    def __init__(self, jobs, k, mode=TQ_MODE_MFMA, query_tile=0, items_per_wg=0):
+    def __init__(self, jobs, k, mode=TQ_MODE_MFMA, query_tile=0, items_per_wg=0,
+                 form=TQ_FORM_AUTO):
         self._jobs = make_jobs(jobs)
         self._keep = jobs  # the spaces must outlive the plan
-        tun = SearchTuning(query_tile, items_per_wg)
+        tun = SearchTuning(query_tile, items_per_wg, form)
         h = ctypes.c_void_p()
         check(lib().tq_search_plan_create(self._jobs, len(jobs), k, mode, ctypes.byref(tun),
                                           ctypes.byref(h)))
@@ -321,7 +324,7 @@ class SearchPlan:
     def tune(self, stream=None):
         t = SearchTuning()
         check(lib().tq_search_plan_tune(self.h, stream, ctypes.byref(t)))
-        return int(t.query_tile), int(t.items_per_wg)
+        return int(t.query_tile), int(t.items_per_wg), int(t.form)
 
     def stats(self):
         items, wgs = ctypes.c_uint64(), ctypes.c_uint64()
