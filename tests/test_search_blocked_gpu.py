@@ -96,6 +96,32 @@ def test_blocked_equals_topk_of_matrix(gpu, big, m):
     np.testing.assert_array_equal(got_d, want_d)
 
 
+@pytest.mark.parametrize("order", ["descending", "ascending"])
+def test_blocked_adversarial_code_orders(gpu, big, order):
+    # Codes sorted by their distance to one query: descending, every chunk
+    # beats the running K-th best of that query's rows (each segment packs
+    # more entries than its list holds and falls back); ascending, no later
+    # chunk has anything at or below it (empty chunk lists).
+    import torch
+    capi = gpu
+    sp, codes, g = big
+    say(f"[gpu] blocked search, codes in {order} distance order")
+    q = torch.randn((3, 128), generator=g, device="cuda")
+    d0 = torch.empty((1, N_BIG), dtype=torch.float32, device="cuda")
+    sp.m_to_n(q[:1].contiguous(), 1, codes, N_BIG, d0, mode=capi.TQ_MODE_MFMA)
+    perm = torch.argsort(d0[0], descending=(order == "descending"), stable=True)
+    sorted_codes = codes.index_select(0, perm).contiguous()
+    del d0, perm
+    Qd = torch.cat([q[:1], q[:1], q[1:]]).contiguous()  # rows 0 and 1 the sorted-for query
+    m = 4
+    want_d, want_i = matrix_topk(torch, capi, sp, Qd, m, sorted_codes, N_BIG, K)
+    (got_d, got_i), = plan_topk(torch, capi, [(sp, Qd, m, sorted_codes, N_BIG)], K)
+    np.testing.assert_array_equal(got_i, want_i)
+    np.testing.assert_array_equal(got_d, want_d)
+    del sorted_codes
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("m", [9, 70])
 def test_blocked_byte_codes(gpu, m):
     # 8-bit codes (the MFMA form's c_lo pass), 32- and 64-query tiles

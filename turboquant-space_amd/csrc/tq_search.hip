@@ -466,12 +466,18 @@ __global__ __launch_bounds__(256) void k_topk_merge(const SearchJobDev *__restri
   wave_pop_topk<K>(td, ti, k_out, lane, job.out_d + q * k_out, job.out_i + q * k_out);
 }
 
-// K pops of the wavefront's best entries from lane-held sorted lists of L
-// entries; lane 0 stores them at (out_d, out_i) as a sorted list.
+// `rounds` (<= K, wavefront-uniform) pops of the wavefront's best entries
+// from lane-held sorted lists of L entries; lane 0 stores them at (out_d,
+// out_i) as a sorted list, and the list's remaining K - rounds entries are
+// set empty (+inf, ~0).
 template <int L, int K>
 __device__ __forceinline__ void wave_pop_list(float (&td)[L], uint32_t (&ti)[L], float *out_d,
-                                              uint32_t *out_i, int lane) {
-  for (int k = 0; k < K; ++k) {
+                                              uint32_t *out_i, int lane, int rounds = K) {
+  if (lane >= rounds && lane < K) {
+    out_d[lane] = __builtin_inff();
+    out_i[lane] = 0xFFFFFFFFu;
+  }
+  for (int k = 0; k < rounds; ++k) {
     float bd = td[0];
     uint32_t bi = ti[0];
     wave_lex_min(bd, bi);
@@ -500,12 +506,17 @@ __device__ __forceinline__ void wave_pop_list(float (&td)[L], uint32_t (&ti)[L],
 // id0).
 //   1. Every lane's minimum over the first sub-block; thr = the K-th smallest
 //      of the 64 minima. They are K distinct entries of the segment, so the
-//      segment's K best all lie at or below thr.
+//      segment's K best all lie at or below thr. With use_run, list 0 of the
+//      row holds the row's K best over the columns before this block (ids
+//      below every id here), and its K-th entry is thr instead when it is
+//      real: nothing above it can be among the row's K best, so the
+//      segment's list keeps only entries at or below it (fewer than K when
+//      few are; the rest of the list is empty).
 //   2. The entries at or below thr (on typical rows a few dozen per
 //      segment) are packed into a per-wavefront LDS list by ballot and lane
 //      prefix counts, sub-block by sub-block.
-//   3. Lanes take them four apiece, sort their four, and K wavefront-minimum
-//      pops give the sorted list. If more than kTopkCap entries lie at or
+//   3. Lanes take them four apiece, sort their four, and min(count, K)
+//      wavefront-minimum pops give the sorted list. If more than kTopkCap entries lie at or
 //      below thr, the segment is read again and each lane sorts its entries
 //      at or below thr into a list of K (exact either way).
 // Loads are unconditional (column index clamped) so a sub-block's loads are
@@ -516,9 +527,10 @@ constexpr int kBufferWord3 = 0x00020000;  // raw 32-bit buffer resource (gfx9 DA
 constexpr int kTopkSeg = 4 * kTopkSub;
 constexpr int kTopkCap = 256;
 template <int K>
-__global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dist, long m, long n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_topk_rows(
+    const float *__restrict__ dist, long m, long n,
                                                    long nseg, long stride, long seg0, uint32_t id0,
-                                                   float *__restrict__ seg_d,
+                                                   int use_run, float *__restrict__ seg_d,
                                                    uint32_t *__restrict__ seg_i) {
   constexpr int PL = kTopkSub / 64;  // columns per lane and sub-block
   __shared__ float c_d[4][kTopkCap];
@@ -562,29 +574,36 @@ __global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dis
     const uint32_t idb = (uint32_t)c0 + id0;            // id of column jl: idb + jl
     float v[PL];
     load_sub(v, rs, 0);
-    // 1. lane minima over the first sub-block, then thr.
-    float md = __builtin_inff();
-    uint32_t mi = 0xFFFFFFFFu;
-#pragma unroll
-    for (int u = 0; u < PL; ++u) {
-      const int jl = u * 64 + lane;
-      const uint32_t id = jl < nl ? idb + (uint32_t)jl : 0xFFFFFFFFu;
-      if (lex_less(v[u], id, md, mi)) {
-        md = v[u];
-        mi = id;
-      }
-    }
+    // 1. thr: the row's running K-th entry, else from the lane minima over
+    // the first sub-block.
     float thr_d = __builtin_inff();
     uint32_t thr_i = 0xFFFFFFFFu;
-    for (int r = 0; r < K; ++r) {
-      float bd = md;
-      uint32_t bi = mi;
-      wave_lex_min(bd, bi);
-      thr_d = bd;
-      thr_i = bi;
-      if (md == bd && mi == bi) {
-        md = __builtin_inff();
-        mi = 0xFFFFFFFFu;
+    if (use_run) {
+      thr_d = seg_d[row * stride * K + K - 1];
+      thr_i = seg_i[row * stride * K + K - 1];
+    }
+    if (thr_i == 0xFFFFFFFFu) {  // wavefront-uniform
+      float md = __builtin_inff();
+      uint32_t mi = 0xFFFFFFFFu;
+#pragma unroll
+      for (int u = 0; u < PL; ++u) {
+        const int jl = u * 64 + lane;
+        const uint32_t id = jl < nl ? idb + (uint32_t)jl : 0xFFFFFFFFu;
+        if (lex_less(v[u], id, md, mi)) {
+          md = v[u];
+          mi = id;
+        }
+      }
+      for (int r = 0; r < K; ++r) {
+        float bd = md;
+        uint32_t bi = mi;
+        wave_lex_min(bd, bi);
+        thr_d = bd;
+        thr_i = bi;
+        if (md == bd && mi == bi) {
+          md = __builtin_inff();
+          mi = 0xFFFFFFFFu;
+        }
       }
     }
     // 2. pack the entries at or below thr, sub-block by sub-block.
@@ -619,7 +638,7 @@ __global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dis
         const int q = lane + 64 * t;
         if (q < cnt) topk_insert<4>(td, ti, c_d[w][q], c_i[w][q]);
       }
-      wave_pop_list<4, K>(td, ti, od, oi, lane);
+      wave_pop_list<4, K>(td, ti, od, oi, lane, cnt < K ? cnt : K);
     } else {
       float td[K];
       uint32_t ti[K];
@@ -805,7 +824,7 @@ hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out
   if (k < 1 || k > kSearchMaxK || m > 65535 || n >= 0xFFFFFFFFL) return hipErrorInvalidValue;
   const long nseg = topk_rows_segments(m, n);
   hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3(topk_rows_grid(m, nseg)), dim3(256), 0, st,
-                     dist, m, n, nseg, nseg, 0L, 0u, seg_d, seg_i);
+                     dist, m, n, nseg, nseg, 0L, 0u, 0, seg_d, seg_i);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_topk_rows_fold<kSearchMaxK>, dim3((unsigned)((m + 3) / 4)), dim3(256), 0,
@@ -814,12 +833,14 @@ hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out
 }
 
 hipError_t launch_topk_rows_chunk(const float *dist, long m, long n, long stride, long seg0,
-                                  uint32_t id0, float *seg_d, uint32_t *seg_i, hipStream_t st) {
+                                  uint32_t id0, bool use_run, float *seg_d, uint32_t *seg_i,
+                                  hipStream_t st) {
   if (m <= 0 || n <= 0) return hipSuccess;
   const long nseg = topk_rows_segments(m, n);
-  if (m > 65535 || n >= 0xFFFFFFFFL || seg0 + nseg > stride) return hipErrorInvalidValue;
+  if (m > 65535 || n >= 0xFFFFFFFFL || seg0 + nseg > stride || (use_run && seg0 < 1))
+    return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3(topk_rows_grid(m, nseg)), dim3(256), 0, st,
-                     dist, m, n, nseg, stride, seg0, id0, seg_d, seg_i);
+                     dist, m, n, nseg, stride, seg0, id0, use_run ? 1 : 0, seg_d, seg_i);
   return hipGetLastError();
 }
 
