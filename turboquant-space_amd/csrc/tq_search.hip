@@ -18,7 +18,10 @@
 // batch it was scored in.
 //
 // k_topk_merge then takes, per query, the best k of the lists of the items
-// that covered it. The reference has no top-k (its users run numpy
+// that covered it. This is the fused form of a search; k_topk_rows and
+// k_topk_rows_fold (further down) give the row-wise top-k of dense distance
+// blocks, which the blocked form (k_mfma blocks over chunks of codes, the
+// plans' default) and the plain fp32 plans use. The reference has no top-k (its users run numpy
 // argpartition on the full distance matrix, python/benchmarks/
 // run_benchmark.py:251-255); ties there are unordered, here they are ordered
 // by id.
