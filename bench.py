@@ -414,9 +414,9 @@ def bench_recall(torch, capi, sh, n=RECALL_N, m=RECALL_Q, k=10):
     del dist, codes
     torch.cuda.empty_cache()
     res = {"vectors": n, "queries": m, "k": k, "dim": DIM, "bits": BITS}
-    for name, ids in (("mfma_fused_search", ids_mfma), ("fp32_distances_topk", ids_fp32)):
+    for name, ids in (("mfma_search_plan", ids_mfma), ("fp32_distances_topk", ids_fp32)):
         res[name] = {"recall_at_1": recall_at_k(ids, gt, 1), "recall_at_10": recall_at_k(ids, gt, k)}
-    log(f"recall@10 vs exact: fused MFMA search {res['mfma_fused_search']['recall_at_10']:.4f}, "
+    log(f"recall@10 vs exact: MFMA search plan {res['mfma_search_plan']['recall_at_10']:.4f}, "
         f"fp32 path {res['fp32_distances_topk']['recall_at_10']:.4f}")
     return res
 

@@ -254,8 +254,8 @@ int tq_topk_rows(const float *dist, uint64_t m, uint64_t n, int k, float *out_di
  * and append requests submitted from any thread. Its scheduler batches
  * continuously: requests that arrive while earlier batches are on the device
  * are admitted as soon as a batch slot and its workspace allow, and all
- * searches of a batch run as one search plan (one persistent launch for
- * TQ_MODE_MFMA). Requests take effect in submission order: a search sees the
+ * searches of a batch run as one search plan (tq_search_plan_create: one
+ * batch per collection). Requests take effect in submission order: a search sees the
  * codes of exactly the appends submitted before it. */
 typedef struct tq_server_s *tq_server;
 

@@ -1,8 +1,10 @@
 """Top-k search on the GPU (tq_search_plan_*, tq_search, tq_topk_rows).
 
 Bars:
-  * the fused MFMA search returns exactly the top-k of the full MFMA distance
-    matrix (same distances bit for bit, ids ordered by (distance, id));
+  * the MFMA search, in both forms (fused persistent launch; blocked distance
+    blocks + row top-k), returns exactly the top-k of the full MFMA distance
+    matrix (same distances bit for bit, ids ordered by (distance, id)), and
+    the two forms agree with each other;
   * the plain fp32 search returns exactly the top-k of the full fp32 matrix,
     and its ids agree with the oracle's top-k except where the oracle's own
     distances are within the fp32 tolerance of each other (near-ties);
