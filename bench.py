@@ -307,8 +307,9 @@ SYM_DIM, SYM_BITS, SYM_N, SYM_M = 1000, 4, 2_000_000, 4096
 
 def bench_symmetric(torch, capi, stream, sh, n=SYM_N, m=SYM_M, rows=512):
     """encode_batch of the 2M base vectors, then distance_m_to_n_symmetric of
-    the 4,096 query codes against them (the centroid-product LUT scan), in
-    blocks of `rows` query codes into one reused [rows, n] output."""
+    the 4,096 query codes against them, in blocks of `rows` query codes into
+    one reused [rows, n] output: by the fp32 centroid-product LUT scan and by
+    the MFMA form (decoded codes as fp16 hi+lo queries, two passes)."""
     sp = capi.Space(SYM_DIM, SYM_BITS)
     codes_b = torch.empty((n, sp.cs), dtype=torch.uint8, device="cuda")
     centres = None
@@ -323,18 +324,24 @@ def bench_symmetric(torch, capi, stream, sh, n=SYM_N, m=SYM_M, rows=512):
         sp.encode(x, x.shape[0], codes_a[start:start + x.shape[0]], stream=sh)
     out = torch.empty((rows, n), dtype=torch.float32, device="cuda")
 
-    def sweep():
+    def sweep(mode):
         for r0 in range(0, m, rows):
             rc = min(rows, m - r0)
-            sp.symmetric(codes_a[r0:r0 + rc], rc, codes_b, n, out, stream=sh)
+            sp.symmetric(codes_a[r0:r0 + rc], rc, codes_b, n, out, stream=sh, mode=mode)
 
-    ms = event_time(torch, stream, sweep, 1)
+    ms = event_time(torch, stream, lambda: sweep(capi.TQ_MODE_FP32), 1)
+    sweep(capi.TQ_MODE_MFMA)
+    ms_mfma = event_time(torch, stream, lambda: sweep(capi.TQ_MODE_MFMA), 1)
     res = {"dim": SYM_DIM, "bits": SYM_BITS, "codes_b": n, "codes_a": m,
            "encode_vec_per_s": n / (enc_ms / 1e3),
            "symmetric_ms": ms, "dist_per_s": m * n / (ms / 1e3),
-           "code_bytes_per_s": m * n * sp.cs / (ms / 1e3)}
-    log(f"symmetric {m} x {n} (dim {SYM_DIM}, {SYM_BITS}-bit): {ms:.1f} ms, "
-        f"{res['dist_per_s'] / 1e9:.2f} G dist/s; encode {res['encode_vec_per_s'] / 1e6:.1f} M vec/s")
+           "code_bytes_per_s": m * n * sp.cs / (ms / 1e3),
+           "mfma": {"ms": ms_mfma, "dist_per_s": m * n / (ms_mfma / 1e3),
+                    "tflops_useful": 2.0 * sp.D * m * n / (ms_mfma / 1e3) / 1e12}}
+    log(f"symmetric {m} x {n} (dim {SYM_DIM}, {SYM_BITS}-bit): fp32 LUT {ms:.1f} ms, "
+        f"{res['dist_per_s'] / 1e9:.2f} G dist/s; MFMA {ms_mfma:.1f} ms, "
+        f"{res['mfma']['dist_per_s'] / 1e9:.1f} G dist/s; "
+        f"encode {res['encode_vec_per_s'] / 1e6:.1f} M vec/s")
     del out, codes_a, codes_b
     torch.cuda.empty_cache()
     return res

@@ -124,14 +124,19 @@ def check_sym(gpu, label, dim, bits, A, B):
     sp.symmetric(dA, len(ca), dB, len(cb), dO)
     torch.cuda.synchronize()
     got = dO.cpu().numpy()
+    sp.symmetric(dA, len(ca), dB, len(cb), dO, mode=capi.TQ_MODE_MFMA)
+    torch.cuda.synchronize()
+    got_m = dO.cpu().numpy()
     sp.close()
     e = scale_rel_err(got, d64, an, bn)
+    e_m = scale_rel_err(got_m, d64, an, bn)
     excess = fp32_parity_excess(got, d32o, d64, an, bn)
     row = (f"{label:34s} m={len(A):4d} n={len(B):6d}  symmetric fp32 max {e.max():.2e}"
-           f" mean {e.mean():.2e}")
+           f" mean {e.mean():.2e}  mfma max {e_m.max():.2e} mean {e_m.mean():.2e}")
     TABLE.append(row)
     say(row)
     assert excess.max() <= FP32_TOL
+    assert e_m.max() <= MFMA_TOL
     return got, d64
 
 

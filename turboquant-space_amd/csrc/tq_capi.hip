@@ -577,7 +577,7 @@ int tq_distance_m_to_n_symmetric_ex(tq_space s, const uint8_t *codes_a, uint64_t
     e = tq::launch_prepare_from_codes(s->dev, da, (long)m, dp, st);
     if (e == hipSuccess) {
       if (mode == TQ_MODE_MFMA) {
-        e = tq::launch_scan_mfma(s->dev, dp, (long)m, db, (long)n, dout, st);
+        e = tq::launch_scan_mfma(s->dev, dp, (long)m, db, (long)n, dout, st, /*qjl=*/false);
         if (e == hipErrorNotSupported)
           return fail(TQ_EUNSUPPORTED, "%s: MFMA mode does not support padded dim %d", what,
                       s->dev.D);

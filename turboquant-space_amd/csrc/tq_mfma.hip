@@ -98,7 +98,10 @@ struct MfmaShape {
 // queries per workgroup (2 x 2 blocks of 32 x 32 per wavefront) or 32 (1 x 2
 // blocks) for batches of at most 32 queries: every output element depends
 // only on its own query row and code column, so both give the same bits.
-template <bool NIB, bool LO, int QT>
+// QJL = false drops the s_q plane and its pass (queries with s_q = 0: the
+// symmetric MSE estimator, whose "queries" are decoded codes); the QJL term
+// is then exactly zero and the distances are what the three-pass form gives.
+template <bool NIB, bool LO, int QT, bool QJL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma(SpaceDev s, const _Float16 *__restrict__ planes,
                                               const float *__restrict__ prep, long m,
                                               const uint8_t *__restrict__ codes, long n,
@@ -107,7 +110,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   static_assert(QT == 32 || QT == 64, "query tile");
   constexpr int RB = QT / 32;                                  // 32-row blocks per wavefront
   constexpr int CROW = Sh::CROW, CPARTS = Sh::CPARTS;
-  constexpr int AUNITS = kPlanes * QT * (kKC / 8);             // 16-byte A units per chunk
+  constexpr int NPL = QJL ? kPlanes : 2;                       // query planes read
+  constexpr int AUNITS = NPL * QT * (kKC / 8);                 // 16-byte A units per chunk
   constexpr int ALOADS = (AUNITS + 255) / 256;                 // per thread (QT = 32: the
                                                                // last unit is loaded twice)
   constexpr int CLOADS = kTN * CPARTS / 256;         // 16-byte code loads per thread
@@ -265,7 +269,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       _Pragma("unroll") for (int c = 0; c < 2; ++c) ACC[r][c] =                          \
           __builtin_amdgcn_mfma_f32_32x32x16_f16(a[r][A], B[c], ACC[r][c], 0, 0, 0);
       TQ_PASS(acc_m, 0, bh)
-      TQ_PASS(acc_q, 2, bt)
+      if constexpr (QJL) {
+        TQ_PASS(acc_q, 2, bt)
+      }
       TQ_PASS(acc_m, 1, bh)
       if constexpr (LO) {
         TQ_PASS(acc_m, 0, bl)
@@ -326,7 +332,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const int row = 32 * r + 8 * (v >> 2) + 4 * kg + (v & 3);
         const long q = q0 + row;
         if (q >= m) continue;
-        const float ip = (acc_m[r][c][v] * fm + acc_q[r][c][v] * fq) * xn * l_qn[row][1];
+        const float ip = (QJL ? acc_m[r][c][v] * fm + acc_q[r][c][v] * fq : acc_m[r][c][v] * fm) *
+                         xn * l_qn[row][1];
         const float d = l_qn[row][0] + xns - 2.0f * ip;
         out[q * n + j] = d > 0.0f ? d : 0.0f;
       }
@@ -345,33 +352,45 @@ hipError_t launch_split_queries(const SpaceDev &s, const float *prep, long m, _F
   return hipGetLastError();
 }
 
+namespace {
+template <int QT, bool QJL>
+void launch_tile(const SpaceDev &s, dim3 grid, const _Float16 *planes, const float *prep, long m,
+                 const uint8_t *codes, long n, float *out, hipStream_t st) {
+  if (s.nib && s.mfma_lo)
+    hipLaunchKernelGGL((k_mfma<true, true, QT, QJL>), grid, dim3(256), 0, st, s, planes, prep, m,
+                       codes, n, out);
+  else if (s.nib)
+    hipLaunchKernelGGL((k_mfma<true, false, QT, QJL>), grid, dim3(256), 0, st, s, planes, prep, m,
+                       codes, n, out);
+  else
+    hipLaunchKernelGGL((k_mfma<false, true, QT, QJL>), grid, dim3(256), 0, st, s, planes, prep, m,
+                       codes, n, out);
+}
+}  // namespace
+
 hipError_t launch_mfma_planes(const SpaceDev &s, const _Float16 *planes, const float *prep, long m,
-                              const uint8_t *codes, long n, float *out, hipStream_t st) {
+                              const uint8_t *codes, long n, float *out, hipStream_t st, bool qjl) {
   if (s.D < kKC || m > 65535L * kTQ) return hipErrorNotSupported;
   if (m <= 0 || n <= 0) return hipSuccess;
   const unsigned gx = (unsigned)((n + kTN - 1) / kTN);
   if (m <= 32) {  // one 32-query tile: half the matrix work of a 64-query one
     const dim3 grid(gx, 1);
-    if (s.nib && s.mfma_lo)
-      hipLaunchKernelGGL((k_mfma<true, true, 32>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
-    else if (s.nib)
-      hipLaunchKernelGGL((k_mfma<true, false, 32>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+    if (qjl)
+      launch_tile<32, true>(s, grid, planes, prep, m, codes, n, out, st);
     else
-      hipLaunchKernelGGL((k_mfma<false, true, 32>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+      launch_tile<32, false>(s, grid, planes, prep, m, codes, n, out, st);
     return hipGetLastError();
   }
   const dim3 grid(gx, (unsigned)((m + kTQ - 1) / kTQ));
-  if (s.nib && s.mfma_lo)
-    hipLaunchKernelGGL((k_mfma<true, true, kTQ>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
-  else if (s.nib)
-    hipLaunchKernelGGL((k_mfma<true, false, kTQ>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+  if (qjl)
+    launch_tile<kTQ, true>(s, grid, planes, prep, m, codes, n, out, st);
   else
-    hipLaunchKernelGGL((k_mfma<false, true, kTQ>), grid, dim3(256), 0, st, s, planes, prep, m, codes, n, out);
+    launch_tile<kTQ, false>(s, grid, planes, prep, m, codes, n, out, st);
   return hipGetLastError();
 }
 
 hipError_t launch_scan_mfma(const SpaceDev &s, const float *prep, long m, const uint8_t *codes,
-                            long n, float *out, hipStream_t st) {
+                            long n, float *out, hipStream_t st, bool qjl) {
   if (s.D < kKC) return hipErrorNotSupported;
   if (m <= 0 || n <= 0) return hipSuccess;
   // grid.y covers 64 queries per workgroup row; larger batches go in chunks.
@@ -392,7 +411,7 @@ hipError_t launch_scan_mfma(const SpaceDev &s, const float *prep, long m, const 
                        planes, s.nib);
     e = hipGetLastError();
     if (e != hipSuccess) break;
-    e = launch_mfma_planes(s, planes, pc, mc, codes, n, oc, st);
+    e = launch_mfma_planes(s, planes, pc, mc, codes, n, oc, st, qjl);
   }
   hipError_t f = hipFreeAsync(planes, st);
   return e != hipSuccess ? e : f;
