@@ -541,18 +541,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   // Within a segment, columns are 32-bit offsets jl < nl from the segment's
-  // first column. The segment is read through a buffer resource of nl * 4
-  // bytes: the hardware returns 0 for offsets past it, so the loads need no
-  // clamping and take a lane offset plus constant offsets (no 64-bit
-  // addresses in registers); columns past nl are then set to +inf.
+  // first column, read through a buffer resource of the segment (a 32-bit
+  // byte offset per lane, no 64-bit addresses in registers). Every offset is
+  // clamped to the segment's last column in the vector offset itself, so no
+  // load leaves the segment whatever the hardware's range check covers;
+  // columns past nl are then set to +inf.
   const float *ps = dist;
   int nl = 0;
   auto load_sub = [&](float (&v)[PL], __amdgpu_buffer_rsrc_t rs, int sb)
                       __attribute__((always_inline)) {
 #pragma unroll
-    for (int u = 0; u < PL; ++u)
-      v[u] = __builtin_bit_cast(
-          float, __builtin_amdgcn_raw_buffer_load_b32(rs, 4 * lane, 4 * (sb * kTopkSub + u * 64), 0));
+    for (int u = 0; u < PL; ++u) {
+      const int jl = sb * kTopkSub + u * 64 + lane;
+      const int jc = jl < nl ? jl : nl - 1;
+      v[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, 4 * jc, 0, 0));
+    }
 #pragma unroll
     for (int u = 0; u < PL; ++u)
       if (sb * kTopkSub + u * 64 + lane >= nl) v[u] = __builtin_inff();
