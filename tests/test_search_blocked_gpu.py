@@ -96,6 +96,22 @@ def test_blocked_equals_topk_of_matrix(gpu, big, m):
     np.testing.assert_array_equal(got_d, want_d)
 
 
+@pytest.mark.parametrize("k", [1, 16])
+def test_blocked_k_bounds(gpu, big, k):
+    # 100 queries over 2.3M codes: four chunks, so later chunks take the
+    # running k-list's last entry (a full list of 16 at k = 16) as their bound.
+    import torch
+    capi = gpu
+    sp, codes, g = big
+    say(f"[gpu] blocked search k={k}")
+    m = 100
+    Qd = torch.randn((m, 128), generator=g, device="cuda")
+    want_d, want_i = matrix_topk(torch, capi, sp, Qd, m, codes, N_BIG, k)
+    (got_d, got_i), = plan_topk(torch, capi, [(sp, Qd, m, codes, N_BIG)], k)
+    np.testing.assert_array_equal(got_i, want_i)
+    np.testing.assert_array_equal(got_d, want_d)
+
+
 @pytest.mark.parametrize("order", ["descending", "ascending"])
 def test_blocked_adversarial_code_orders(gpu, big, order):
     # Codes sorted by their distance to one query: descending, every chunk

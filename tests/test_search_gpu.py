@@ -118,6 +118,26 @@ def test_mfma_search_equals_topk_of_mfma_matrix(gpu, torch_cuda, dim, bits, form
         plan.close()
 
 
+@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("k", [1, 16])
+def test_mfma_search_k_bounds(gpu, torch_cuda, form, k):
+    # k = 1 and k = 16 (the list length): the blocked form's running bound is
+    # then the K-th entry of a full list, and k = 1 pops a single entry.
+    torch, capi = torch_cuda, gpu
+    say(f"[gpu] MFMA search form={form} k={k}")
+    n, m = 9000, 33
+    c = Coll(torch, capi, 128, 4, n, seed=90 + k + form, dup=300)
+    Q, dQ = c.queries(torch, m)
+    od, oi = outputs(torch, m, k)
+    plan = capi.SearchPlan([(c.sp, dQ, m, c.codes, n, od, oi)], k, capi.TQ_MODE_MFMA, form=form)
+    plan.run()
+    torch.cuda.synchronize()
+    want_d, want_i = host_topk(full_matrix(torch, capi, c, dQ, m, capi.TQ_MODE_MFMA), k)
+    np.testing.assert_array_equal(oi[:m].cpu().numpy(), want_i)
+    np.testing.assert_array_equal(od[:m].cpu().numpy().view(np.uint32), want_d.view(np.uint32))
+    plan.close()
+
+
 @pytest.mark.parametrize("dim,bits", [(128, 4), (768, 4), (128, 8), (64, 2)])
 def test_fp32_search_equals_topk_of_fp32_matrix_and_oracle(gpu, torch_cuda, dim, bits):
     torch, capi = torch_cuda, gpu
