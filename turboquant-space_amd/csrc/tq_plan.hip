@@ -505,8 +505,8 @@ int enqueue(tq_search_plan_s *p, hipStream_t st) {
         const bool last = c0 + nc >= b.n;
         if ((e = tq::launch_mfma_planes(b.space->dev, b.planes, b.prep, b.m, b.codes + c0 * cs, nc,
                                         p->dist, st)) != hipSuccess ||
-            (e = tq::launch_topk_rows_chunk(p->dist, b.m, nc, b.lists, 1, (uint32_t)c0, c0 > 0,
-                                            p->seg_d, p->seg_i, st)) != hipSuccess ||
+            (e = tq::launch_topk_rows_chunk(p->dist, b.m, nc, b.lists, 1, (uint32_t)c0, p->k,
+                                            c0 > 0, p->seg_d, p->seg_i, st)) != hipSuccess ||
             (e = tq::launch_topk_rows_fold(b.m, b.lists, c0 == 0 ? 1 : 0, l1, p->k, p->seg_d,
                                            p->seg_i, last ? p->all_d + b.row0 * p->k : nullptr,
                                            last ? p->all_i + b.row0 * p->k : nullptr, st)) !=

@@ -50,15 +50,15 @@ hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out
 // The same over one chunk of columns, kept as partial lists: the
 // topk_rows_segments(m, n) sorted lists of each row of the [m][n] block go to
 // lists seg0 .. of that row's `stride` lists (entries of kSearchMaxK), with
-// ids = column + id0. With use_run (seg0 >= 1), list 0 of each row holds
-// that row's best kSearchMaxK over earlier chunks (smaller ids), and the
-// chunk's lists keep only entries at or below its last real entry.
-// launch_topk_rows_fold then merges lists [l0, l1) of every row either into
-// list 0 (out_d == nullptr) or into the final k per row at out_d / out_i
-// (ids ~0 -> -1).
+// ids = column + id0; a list keeps its segment's best k (the rest empty).
+// With use_run (seg0 >= 1), list 0 of each row holds that row's best k over
+// earlier chunks (smaller ids), and the chunk's lists keep only entries at
+// or below its k-th entry. launch_topk_rows_fold then merges lists [l0, l1)
+// of every row either into list 0 (out_d == nullptr) or into the final k per
+// row at out_d / out_i (ids ~0 -> -1).
 hipError_t launch_topk_rows_chunk(const float *dist, long m, long n, long stride, long seg0,
-                                  uint32_t id0, bool use_run, float *seg_d, uint32_t *seg_i,
-                                  hipStream_t st);
+                                  uint32_t id0, int k, bool use_run, float *seg_d,
+                                  uint32_t *seg_i, hipStream_t st);
 hipError_t launch_topk_rows_fold(long m, long stride, long l0, long l1, int k, float *seg_d,
                                  uint32_t *seg_i, float *out_d, int64_t *out_i, hipStream_t st);
 

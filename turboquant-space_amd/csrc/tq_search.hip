@@ -505,20 +505,21 @@ __device__ __forceinline__ void wave_pop_list(float (&td)[L], uint32_t (&ti)[L],
 // in turn (wavefront w of workgroup x takes pairs 4x + w, + 4 * gridDim.x,
 // ...; rows shorter than four segments share a workgroup), each segment read
 // as sub-blocks of kTopkSub columns (32 per lane, in registers); a segment's
-// best K go to list seg0 + seg of its row's `stride` lists (ids = column +
-// id0).
-//   1. Every lane's minimum over the first sub-block; thr = the K-th smallest
-//      of the 64 minima. They are K distinct entries of the segment, so the
-//      segment's K best all lie at or below thr. With use_run, list 0 of the
-//      row holds the row's K best over the columns before this block (ids
-//      below every id here), and its K-th entry is thr instead when it is
-//      real: nothing above it can be among the row's K best, so the
-//      segment's list keeps only entries at or below it (fewer than K when
-//      few are; the rest of the list is empty).
+// best kk (<= K; the rest of the list empty) go to list seg0 + seg of its
+// row's `stride` lists of K entries (ids = column + id0). Only a row's kk
+// best are ever asked for, and they are among its segments' kk best.
+//   1. Every lane's minimum over the first sub-block; thr = the kk-th
+//      smallest of the 64 minima. They are kk distinct entries of the
+//      segment, so the segment's kk best all lie at or below thr. With
+//      use_run, list 0 of the row holds the row's kk best over the columns
+//      before this block (ids below every id here), and its kk-th entry is
+//      thr instead when it is real: nothing above it can be among the row's
+//      kk best, so the segment's list keeps only entries at or below it
+//      (fewer than kk when few are).
 //   2. The entries at or below thr (on typical rows a few dozen per
 //      segment) are packed into a per-wavefront LDS list by ballot and lane
 //      prefix counts, sub-block by sub-block.
-//   3. Lanes take them four apiece, sort their four, and min(count, K)
+//   3. Lanes take them four apiece, sort their four, and min(count, kk)
 //      wavefront-minimum pops give the sorted list. If more than kTopkCap entries lie at or
 //      below thr, the segment is read again and each lane sorts its entries
 //      at or below thr into a list of K (exact either way).
@@ -533,7 +534,7 @@ template <int K>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_topk_rows(
     const float *__restrict__ dist, long m, long n,
                                                    long nseg, long stride, long seg0, uint32_t id0,
-                                                   int use_run, float *__restrict__ seg_d,
+                                                   int kk, int use_run, float *__restrict__ seg_d,
                                                    uint32_t *__restrict__ seg_i) {
   constexpr int PL = kTopkSub / 64;  // columns per lane and sub-block
   __shared__ float c_d[4][kTopkCap];
@@ -580,13 +581,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const uint32_t idb = (uint32_t)c0 + id0;            // id of column jl: idb + jl
     float v[PL];
     load_sub(v, rs, 0);
-    // 1. thr: the row's running K-th entry, else from the lane minima over
+    // 1. thr: the row's running kk-th entry, else from the lane minima over
     // the first sub-block.
     float thr_d = __builtin_inff();
     uint32_t thr_i = 0xFFFFFFFFu;
     if (use_run) {
-      thr_d = seg_d[row * stride * K + K - 1];
-      thr_i = seg_i[row * stride * K + K - 1];
+      thr_d = seg_d[row * stride * K + kk - 1];
+      thr_i = seg_i[row * stride * K + kk - 1];
     }
     if (thr_i == 0xFFFFFFFFu) {  // wavefront-uniform
       float md = __builtin_inff();
@@ -600,7 +601,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           mi = id;
         }
       }
-      for (int r = 0; r < K; ++r) {
+      for (int r = 0; r < kk; ++r) {
         float bd = md;
         uint32_t bi = mi;
         wave_lex_min(bd, bi);
@@ -644,7 +645,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const int q = lane + 64 * t;
         if (q < cnt) topk_insert<4>(td, ti, c_d[w][q], c_i[w][q]);
       }
-      wave_pop_list<4, K>(td, ti, od, oi, lane, cnt < K ? cnt : K);
+      wave_pop_list<4, K>(td, ti, od, oi, lane, cnt < kk ? cnt : kk);
     } else {
       float td[K];
       uint32_t ti[K];
@@ -656,7 +657,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const uint32_t id = idb + (uint32_t)jl;
         if (!lex_less(thr_d, thr_i, d, id)) topk_insert<K>(td, ti, d, id);
       }
-      wave_pop_list<K, K>(td, ti, od, oi, lane);
+      wave_pop_list<K, K>(td, ti, od, oi, lane, kk);
     }
     // the next segment's stores to this LDS list must follow these reads
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -666,12 +667,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // Stage 2: merges lists [l0, l1) of each row (`stride` sorted lists of K
 // per row), one wavefront per row, the same way stage 1 treats columns:
-// thr = the K-th smallest of the 64 lanes' smallest list heads (K distinct
-// entries, so the row's K best lie at or below it); the entries at or below
-// thr (a walk of each list that starts at or below thr) are packed into an
-// LDS list by an LDS counter, four per lane are sorted and K wavefront
-// pops give the result: the row's best k_out into out_d / out_i (when out_d
-// is set, ids ~0 -> -1) or its best K back into list 0 as a sorted list.
+// thr = the k_out-th smallest of the 64 lanes' smallest list heads (k_out
+// distinct entries, so the row's k_out best lie at or below it); the
+// entries at or below thr (a walk of each list that starts at or below thr)
+// are packed into an LDS list by an LDS counter, four per lane are sorted
+// and wavefront pops give the result: the row's best k_out into out_d /
+// out_i (when out_d is set, ids ~0 -> -1) or back into list 0 as a sorted
+// list (the rest of it empty).
 // More than kTopkCap such entries: every lane folds its lists' entries at or
 // below thr into a list of K instead (exact either way). Only lane 0 reads
 // list 0 and it does before any pop writes it.
@@ -715,7 +717,7 @@ __global__ __launch_bounds__(256) void k_topk_rows_fold(long m, long stride, lon
   }
   float thr_d = __builtin_inff();
   uint32_t thr_i = 0xFFFFFFFFu;
-  for (int r = 0; r < K; ++r) {
+  for (int r = 0; r < k_out; ++r) {
     float bd = md;
     uint32_t bi = mi;
     wave_lex_min(bd, bi);
@@ -759,7 +761,8 @@ __global__ __launch_bounds__(256) void k_topk_rows_fold(long m, long stride, lon
     if (out_d) {
       wave_pop_topk<4>(td, ti, k_out, lane, out_d + row * k_out, out_i + row * k_out);
     } else {
-      wave_pop_list<4, K>(td, ti, seg_d + row * stride * K, seg_i + row * stride * K, lane);
+      wave_pop_list<4, K>(td, ti, seg_d + row * stride * K, seg_i + row * stride * K, lane,
+                          cnt < k_out ? cnt : k_out);
     }
   } else {
     float td[K];
@@ -770,7 +773,7 @@ __global__ __launch_bounds__(256) void k_topk_rows_fold(long m, long stride, lon
     if (out_d) {
       wave_pop_topk<K>(td, ti, k_out, lane, out_d + row * k_out, out_i + row * k_out);
     } else {
-      wave_pop_list<K, K>(td, ti, seg_d + row * stride * K, seg_i + row * stride * K, lane);
+      wave_pop_list<K, K>(td, ti, seg_d + row * stride * K, seg_i + row * stride * K, lane, k_out);
     }
   }
 }
@@ -830,7 +833,7 @@ hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out
   if (k < 1 || k > kSearchMaxK || m > 65535 || n >= 0xFFFFFFFFL) return hipErrorInvalidValue;
   const long nseg = topk_rows_segments(m, n);
   hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3(topk_rows_grid(m, nseg)), dim3(256), 0, st,
-                     dist, m, n, nseg, nseg, 0L, 0u, 0, seg_d, seg_i);
+                     dist, m, n, nseg, nseg, 0L, 0u, k, 0, seg_d, seg_i);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_topk_rows_fold<kSearchMaxK>, dim3((unsigned)((m + 3) / 4)), dim3(256), 0,
@@ -839,14 +842,15 @@ hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out
 }
 
 hipError_t launch_topk_rows_chunk(const float *dist, long m, long n, long stride, long seg0,
-                                  uint32_t id0, bool use_run, float *seg_d, uint32_t *seg_i,
-                                  hipStream_t st) {
+                                  uint32_t id0, int k, bool use_run, float *seg_d,
+                                  uint32_t *seg_i, hipStream_t st) {
   if (m <= 0 || n <= 0) return hipSuccess;
   const long nseg = topk_rows_segments(m, n);
-  if (m > 65535 || n >= 0xFFFFFFFFL || seg0 + nseg > stride || (use_run && seg0 < 1))
+  if (k < 1 || k > kSearchMaxK || m > 65535 || n >= 0xFFFFFFFFL || seg0 + nseg > stride ||
+      (use_run && seg0 < 1))
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3(topk_rows_grid(m, nseg)), dim3(256), 0, st,
-                     dist, m, n, nseg, stride, seg0, id0, use_run ? 1 : 0, seg_d, seg_i);
+                     dist, m, n, nseg, stride, seg0, id0, k, use_run ? 1 : 0, seg_d, seg_i);
   return hipGetLastError();
 }
 
