@@ -740,7 +740,8 @@ int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen
         for (int ipw : {1, 2, 4, 8}) cands.push_back(tq_search_tuning{qt, ipw, TQ_FORM_FUSED});
       if (b.space->info.padded_dim >= 32 && tq::plan_blocked(b.n, b.m))
         cands.push_back(tq_search_tuning{0, 0, TQ_FORM_BLOCKED});
-      for (const tq_search_tuning &cand : cands) {
+      for (size_t ci = 0; ci < cands.size(); ++ci) {
+        const tq_search_tuning &cand = cands[ci];
         tq_search_plan sp = nullptr;
         rc = tq_search_plan_create(sub.data(), sub.size(), p->k, p->mode, &cand, &sp);
         if (rc) break;
@@ -759,7 +760,7 @@ int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen
           best = cand;
           best_ms = run_best;
         }
-        if (&cand == &cands[0] || run_best < best_form_ms) {
+        if (ci == 0 || run_best < best_form_ms) {
           best_form = cand.form;
           best_form_ms = run_best;
         }
