@@ -96,6 +96,31 @@ def test_blocked_equals_topk_of_matrix(gpu, big, m):
     np.testing.assert_array_equal(got_d, want_d)
 
 
+def test_blocked_at_the_served_size(gpu):
+    # The serving and headline collection size: 10,000,000 codes at dim 768,
+    # 4-bit; 64 queries take ten 1M-code chunks. The plan's top-10 equals the
+    # top-10 of the full [64, 10M] MFMA matrix, distances bit for bit.
+    import torch
+    capi = gpu
+    say("[gpu] blocked search over 10M codes, dim 768 4-bit, m=64")
+    sp = capi.Space(768, 4)
+    n, m = 10_000_000, 64
+    g = torch.Generator(device="cuda").manual_seed(23)
+    codes = torch.empty((n, sp.cs), dtype=torch.uint8, device="cuda")
+    for s0 in range(0, n, 1_000_000):
+        x = torch.randn((1_000_000, 768), generator=g, device="cuda")
+        sp.encode(x, 1_000_000, codes[s0:s0 + 1_000_000])
+    del x
+    Qd = torch.randn((m, 768), generator=g, device="cuda")
+    want_d, want_i = matrix_topk(torch, capi, sp, Qd, m, codes, n, K)
+    (got_d, got_i), = plan_topk(torch, capi, [(sp, Qd, m, codes, n)], K)
+    np.testing.assert_array_equal(got_i, want_i)
+    np.testing.assert_array_equal(got_d, want_d)
+    del codes
+    torch.cuda.empty_cache()
+    sp.close()
+
+
 @pytest.mark.parametrize("k", [1, 16])
 def test_blocked_k_bounds(gpu, big, k):
     # 100 queries over 2.3M codes: four chunks, so later chunks take the
