@@ -693,9 +693,11 @@ __global__ __launch_bounds__(256) void k_topk_rows_fold(long m, long stride, lon
   const float *rd = seg_d + row * stride * K;
   const uint32_t *ri = seg_i + row * stride * K;
   const long nl = l1 - l0;
-  // 1. the lane's smallest head; thr.
+  // 1. the lane's smallest head; thr. A row whose lists past list 0 are all
+  // empty (later chunks of a blocked search, mostly) keeps list 0 as it is.
   float md = __builtin_inff();
   uint32_t mi = 0xFFFFFFFFu;
+  bool fresh = false;  // a real entry in a list other than list 0
   for (long b = 0; b < nl; b += 64 * 8) {
     float hd[8];
     uint32_t hi[8];
@@ -709,12 +711,14 @@ __global__ __launch_bounds__(256) void k_topk_rows_fold(long m, long stride, lon
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const long l = b + t * 64 + lane;
+      if (l < nl && l0 + l != 0 && hi[t] != 0xFFFFFFFFu) fresh = true;
       if (l < nl && lex_less(hd[t], hi[t], md, mi)) {
         md = hd[t];
         mi = hi[t];
       }
     }
   }
+  if (!out_d && !__any(fresh)) return;  // wavefront-uniform; no barrier below
   float thr_d = __builtin_inff();
   uint32_t thr_i = 0xFFFFFFFFu;
   for (int r = 0; r < k_out; ++r) {
