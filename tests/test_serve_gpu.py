@@ -198,3 +198,29 @@ def test_admission_limits(gpu, torch_cuda):
     with pytest.raises(capi.TQError, match="max_batch_queries"):
         srv.search(c, Q, od, oi)
     srv.close()
+
+
+def test_add_collections_while_searches_run(gpu, torch_cuda):
+    """Collections added while batches are in flight must not disturb the
+    searches already submitted (the scheduler reads each request's collection
+    without the server lock)."""
+    torch, capi = torch_cuda, gpu
+    say("[gpu] server add collections during searches")
+    rng = np.random.default_rng(11)
+    srv, colls = make_server(capi, capi.TQ_MODE_MFMA, SPECS[:1], rng, max_batch_queries=64)
+    c, sp, centres, _ = colls[0]
+    reqs = [gm(rng, int(rng.integers(1, 48)), sp.dim, centres) for _ in range(40)]
+    outs = [(np.zeros((len(Q), 10), np.float32), np.zeros((len(Q), 10), np.int64))
+            for Q in reqs]
+    tickets = [srv.search(c, Q, *o) for Q, o in zip(reqs, outs)]
+    extra = [srv.add_collection(capi.Space(128, 4), 16) for _ in range(64)]
+    assert len(set(extra)) == 64
+    for t in tickets:
+        srv.wait(t)
+    n, codes = srv.collection(c)
+    for Q, (od, oi) in zip(reqs, outs):
+        pd, pi = plan_topk(torch, capi, sp, codes, n, Q, 10, capi.TQ_MODE_MFMA)
+        np.testing.assert_array_equal(oi, pi)
+        np.testing.assert_array_equal(od.view(np.uint32), pd.view(np.uint32))
+    assert srv.stats()["failed"] == 0
+    srv.close()

@@ -645,8 +645,10 @@ bool plan_blocked(long n, long rows) {
 size_t plan_blocked_ws_bound(long rows, long n_max) {
   // The distance block: rows x nc floats, nc <= min(n, max(4096,
   // kMfmaBlockBytes / (4 * rows))); and (1 + segments) lists per query over a
-  // chunk, segments <= ceil(nc / 2048): rows * lists <= 4 * rows +
-  // kMfmaBlockBytes / 8192.
+  // chunk, segments = topk_rows_segments(rows, nc) = ceil(nc / 8192):
+  // rows * lists <= 2 * rows + rows * nc / 8192, which the conservative
+  // 4 * rows + kMfmaBlockBytes / 8192 covers (rows * nc * 4 bytes is at most
+  // max(kMfmaBlockBytes, 4096 * 4 * rows)).
   const size_t r = (size_t)std::max(rows, 1L);
   const size_t block = std::min(std::max(kMfmaBlockBytes, r * 4096 * sizeof(float)),
                                 r * (size_t)std::max(n_max, 1L) * sizeof(float));

@@ -718,7 +718,10 @@ __global__ __launch_bounds__(256) void k_topk_rows_fold(long m, long stride, lon
       }
     }
   }
-  if (!out_d && !__any(fresh)) return;  // wavefront-uniform; no barrier below
+  // Nothing new to fold into list 0: keep it as it is. Only when list 0 is
+  // one of the inputs (l0 == 0); otherwise list 0 is this fold's output and
+  // is always written. Wavefront-uniform; no barrier below.
+  if (!out_d && l0 == 0 && !__any(fresh)) return;
   float thr_d = __builtin_inff();
   uint32_t thr_i = 0xFFFFFFFFu;
   for (int r = 0; r < k_out; ++r) {

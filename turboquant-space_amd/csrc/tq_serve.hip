@@ -45,6 +45,11 @@ struct Request {
   uint64_t id = 0;
   Kind kind = kSearch;
   uint32_t coll = 0;
+  // The collection's space and code buffer, copied at submission under the
+  // server lock: the scheduler reads them without the lock while other
+  // threads may add collections.
+  tq_space space = nullptr;
+  uint8_t *codes = nullptr;
   const float *X = nullptr;  // queries or vectors (host)
   uint64_t rows = 0;
   uint64_t n_at = 0;  // search: codes visible to it; append: first row it writes
@@ -90,7 +95,8 @@ struct tq_server_s {
   std::condition_variable cv_sched, cv_done, cv_complete;
   std::deque<Request *> pending;
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
-  std::vector<Collection> colls;
+  // A deque: adding a collection leaves the others where they are.
+  std::deque<Collection> colls;
   std::vector<Slot> slots;
   std::deque<Slot *> inflight;
   Slot *last = nullptr;  // slot of the most recently enqueued batch
@@ -136,7 +142,7 @@ int enqueue_batch(tq_server_s *s, Slot &sl, Slot *prev) {
   std::vector<size_t> in_off;
   for (Request *r : sl.reqs) {
     in_off.push_back(in_bytes);
-    in_bytes += align_up(r->rows * s->colls[r->coll].space->info.dim * sizeof(float));
+    in_bytes += align_up(r->rows * r->space->info.dim * sizeof(float));
     if (r->kind == kSearch) nres += r->rows * k;
   }
   const size_t o_res_d = in_bytes;
@@ -151,8 +157,7 @@ int enqueue_batch(tq_server_s *s, Slot &sl, Slot *prev) {
   for (size_t t = 0; t < sl.reqs.size(); ++t) {
     Request *r = sl.reqs[t];
     if (r->kind != kSearch) continue;
-    const Collection &c = s->colls[r->coll];
-    jobs.push_back({c.space, reinterpret_cast<const float *>(dws + in_off[t]), r->rows, c.codes,
+    jobs.push_back({r->space, reinterpret_cast<const float *>(dws + in_off[t]), r->rows, r->codes,
                     r->n_at, reinterpret_cast<float *>(dws + o_res_d) + res,
                     reinterpret_cast<int64_t *>(dws + o_res_i) + res});
     sl.res_off.push_back(res);
@@ -174,8 +179,7 @@ int enqueue_batch(tq_server_s *s, Slot &sl, Slot *prev) {
   char *pin = static_cast<char *>(sl.pin);
   for (size_t t = 0; t < sl.reqs.size(); ++t) {
     Request *r = sl.reqs[t];
-    std::memcpy(pin + in_off[t], r->X,
-                r->rows * s->colls[r->coll].space->info.dim * sizeof(float));
+    std::memcpy(pin + in_off[t], r->X, r->rows * r->space->info.dim * sizeof(float));
   }
   sl.res_d = o_res_d;
   sl.res_i = o_res_i;
@@ -189,9 +193,8 @@ int enqueue_batch(tq_server_s *s, Slot &sl, Slot *prev) {
   for (size_t t = 0; t < sl.reqs.size(); ++t) {
     Request *r = sl.reqs[t];
     if (r->kind != kAppend || r->rows == 0) continue;
-    const Collection &c = s->colls[r->coll];
-    e = tq::launch_encode(c.space->dev, reinterpret_cast<const float *>(dws + in_off[t]),
-                          (long)r->rows, c.codes + r->n_at * c.space->info.code_size, sl.st);
+    e = tq::launch_encode(r->space->dev, reinterpret_cast<const float *>(dws + in_off[t]),
+                          (long)r->rows, r->codes + r->n_at * r->space->info.code_size, sl.st);
     if (e != hipSuccess) return hip_fail(e, what);
   }
   if ((e = hipEventRecord(sl.fence, sl.st)) != hipSuccess) return hip_fail(e, what);
@@ -220,7 +223,7 @@ size_t batch_bound(tq_server_s *s, const std::vector<Request *> &b) {
   size_t qtiles = 0;
   bool any = false;
   for (Request *r : b) {
-    const tq_space_info &in = s->colls[r->coll].space->info;
+    const tq_space_info &in = r->space->info;
     bytes += align_up(r->rows * in.dim * sizeof(float)) + 2048;
     if (r->kind != kSearch) continue;
     any = true;
@@ -501,6 +504,8 @@ int tq_server_submit_append(tq_server s, uint32_t coll, const float *X, uint64_t
     return fail(TQ_ENOMEM, "%s: %llu vectors exceed the batch workspace", what,
                 (unsigned long long)n);
   Request *r = new_request(s, kAppend, coll, X, n);
+  r->space = c.space;
+  r->codes = c.codes;
   r->n_at = c.n;
   c.n += n;
   s->pending.push_back(r);
@@ -520,6 +525,8 @@ int tq_server_submit_search(tq_server s, uint32_t coll, const float *Q, uint64_t
     return fail(TQ_EINVAL, "%s: %llu queries exceed max_batch_queries %u", what,
                 (unsigned long long)m, s->cfg.max_batch_queries);
   Request *r = new_request(s, kSearch, coll, Q, m);
+  r->space = s->colls[coll].space;
+  r->codes = s->colls[coll].codes;
   r->n_at = s->colls[coll].n;
   r->out_d = out_dist;
   r->out_i = out_ids;
