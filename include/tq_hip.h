@@ -178,8 +178,12 @@ int tq_encode_graph_destroy(tq_encode_graph g);
  *
  * A job is one query batch against one collection; jobs of one call may use
  * different spaces (dims, bit widths) on the same device. All job buffers are
- * device memory. */
-#define TQ_SEARCH_MAX_K 16
+ * device memory. k is at most TQ_SEARCH_MAX_K (the reference's recall
+ * harness asks for up to 100, python/benchmarks/run_benchmark.py:329-337);
+ * the fused form keeps its lists in registers and takes k <= TQ_FUSED_MAX_K,
+ * so larger k always runs blocked. */
+#define TQ_SEARCH_MAX_K 128
+#define TQ_FUSED_MAX_K 16
 
 typedef struct tq_search_job {
   tq_space space;
@@ -217,8 +221,10 @@ typedef struct tq_search_plan_s *tq_search_plan;
  * or 4,096 columns, a row top-k per chunk, candidates folded per query), or
  * fused (every fused batch of the plan in one persistent launch over a
  * (query tile, code range) queue, the top-k lists kept in registers); both
- * give the same results. TQ_MODE_FP32: per job the plain fp32 distance
- * matrix in row blocks, then a row top-k. `tuning` may be NULL. */
+ * give the same results. k > TQ_FUSED_MAX_K always runs blocked (a fused
+ * `tuning` is then refused, as are MFMA plans of more than 65535 queries).
+ * TQ_MODE_FP32: per job the plain fp32 distance matrix in row blocks, then a
+ * row top-k. `tuning` may be NULL. */
 int tq_search_plan_create(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
                           const tq_search_tuning *tuning, tq_search_plan *out);
 /* Enqueues the plan on `stream`: query preparation, the scan and the top-k.

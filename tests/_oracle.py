@@ -171,3 +171,38 @@ def fp32_parity_excess(got, oracle32, exact, qns, xns):
     same-sign sums (self-distances at large D) its own error approaches 1e-5,
     while the GPU's tree-ordered sums stay well below it."""
     return (scale_rel_err(got, oracle32, qns, xns) - scale_rel_err(oracle32, exact, qns, xns))
+
+
+# Floor of the relative error |d - d64| / max(d64, eps): eps = EPS_FLOOR *
+# (||q||^2 + ||x||^2), i.e. distances below 0.1% of their scale count as that.
+EPS_FLOOR = 1e-3
+
+
+def eps_rel_err(got, want, qns, xns, floor=EPS_FLOOR):
+    """|got - want| / max(want, floor * (||q||^2 + ||x||^2)), elementwise."""
+    scale = np.asarray(qns)[:, None] + np.asarray(xns)[None, :]
+    denom = np.maximum(want.astype(np.float64), floor * scale)
+    denom = np.maximum(denom, 1e-30)
+    return np.abs(got.astype(np.float64) - want.astype(np.float64)) / denom
+
+
+def nearest(d, k=10):
+    """[m, k] column indices of each row's k smallest entries (ascending)."""
+    idx = np.argpartition(d, k, axis=1)[:, :k]
+    order = np.argsort(np.take_along_axis(d, idx, axis=1), axis=1, kind="stable")
+    return np.take_along_axis(idx, order, axis=1)
+
+
+def topk_rel_err(got, want, k=10, idx=None, floor=None):
+    """|got - want| / want on each row's k nearest columns by `want` (or the
+    columns `idx`): the relative error of the distances that decide a top-k.
+    With `floor` ([m, n], e.g. EPS_FLOOR * (||q||^2 + ||x||^2)) the
+    denominator is max(want, floor) (exact duplicates have want = 0)."""
+    if idx is None:
+        idx = nearest(want, k)
+    g = np.take_along_axis(got.astype(np.float64), idx, axis=1)
+    w = np.take_along_axis(want.astype(np.float64), idx, axis=1)
+    den = np.maximum(w, 1e-30)
+    if floor is not None:
+        den = np.maximum(den, np.take_along_axis(np.asarray(floor, np.float64), idx, axis=1))
+    return np.abs(g - w) / den

@@ -77,7 +77,7 @@ def test_search_entry_points_validate_before_device_use():
     assert L.tq_search_plan_create(jobs, 1, 10, capi.TQ_MODE_MFMA, ctypes.byref(bad_form),
                                    ctypes.byref(h)) == capi.TQ_EINVAL
     assert b"form must be" in L.tq_last_error()
-    assert L.tq_topk_rows(None, 1, 1, 17, None, None, None) == capi.TQ_EINVAL
+    assert L.tq_topk_rows(None, 1, 1, 129, None, None, None) == capi.TQ_EINVAL
     assert L.tq_topk_rows(None, 70000, 1, 10, None, None, None) == capi.TQ_EINVAL
     assert L.tq_search_plan_run(None, None) == capi.TQ_EINVAL
     assert L.tq_search_plan_destroy(None) == capi.TQ_OK

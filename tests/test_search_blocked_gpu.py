@@ -55,7 +55,7 @@ def matrix_topk(torch, capi, sp, Qd, m, codes, n, k):
     return out
 
 
-def plan_topk(torch, capi, jobs, k, graph_replays=0):
+def plan_topk(torch, capi, jobs, k, graph_replays=0, mode=None):
     outs = []
     spec = []
     for sp, Qd, m, codes, n in jobs:
@@ -63,7 +63,7 @@ def plan_topk(torch, capi, jobs, k, graph_replays=0):
         oi = torch.empty((m, k), dtype=torch.int64, device="cuda")
         outs.append((od, oi))
         spec.append((sp, Qd, m, codes, n, od, oi))
-    plan = capi.SearchPlan(spec, k)
+    plan = capi.SearchPlan(spec, k) if mode is None else capi.SearchPlan(spec, k, mode)
     plan.run()
     torch.cuda.synchronize()
     res = [(od.cpu().numpy(), oi.cpu().numpy()) for od, oi in outs]
@@ -121,10 +121,11 @@ def test_blocked_at_the_served_size(gpu):
     sp.close()
 
 
-@pytest.mark.parametrize("k", [1, 16])
+@pytest.mark.parametrize("k", [1, 16, 17, 100, 128])
 def test_blocked_k_bounds(gpu, big, k):
     # 100 queries over 2.3M codes: four chunks, so later chunks take the
-    # running k-list's last entry (a full list of 16 at k = 16) as their bound.
+    # running k-list's last entry as their bound. k = 100 is the reference
+    # harness's recall@100 (run_benchmark.py:329-337); k > 16 runs blocked only.
     import torch
     capi = gpu
     sp, codes, g = big
@@ -137,8 +138,9 @@ def test_blocked_k_bounds(gpu, big, k):
     np.testing.assert_array_equal(got_d, want_d)
 
 
+@pytest.mark.parametrize("k", [10, 100])
 @pytest.mark.parametrize("order", ["descending", "ascending"])
-def test_blocked_adversarial_code_orders(gpu, big, order):
+def test_blocked_adversarial_code_orders(gpu, big, order, k):
     # Codes sorted by their distance to one query: descending, every chunk
     # beats the running K-th best of that query's rows (each segment packs
     # more entries than its list holds and falls back); ascending, no later
@@ -146,7 +148,7 @@ def test_blocked_adversarial_code_orders(gpu, big, order):
     import torch
     capi = gpu
     sp, codes, g = big
-    say(f"[gpu] blocked search, codes in {order} distance order")
+    say(f"[gpu] blocked search, codes in {order} distance order, k={k}")
     q = torch.randn((3, 128), generator=g, device="cuda")
     d0 = torch.empty((1, N_BIG), dtype=torch.float32, device="cuda")
     sp.m_to_n(q[:1].contiguous(), 1, codes, N_BIG, d0, mode=capi.TQ_MODE_MFMA)
@@ -155,8 +157,8 @@ def test_blocked_adversarial_code_orders(gpu, big, order):
     del d0, perm
     Qd = torch.cat([q[:1], q[:1], q[1:]]).contiguous()  # rows 0 and 1 the sorted-for query
     m = 4
-    want_d, want_i = matrix_topk(torch, capi, sp, Qd, m, sorted_codes, N_BIG, K)
-    (got_d, got_i), = plan_topk(torch, capi, [(sp, Qd, m, sorted_codes, N_BIG)], K)
+    want_d, want_i = matrix_topk(torch, capi, sp, Qd, m, sorted_codes, N_BIG, k)
+    (got_d, got_i), = plan_topk(torch, capi, [(sp, Qd, m, sorted_codes, N_BIG)], k)
     np.testing.assert_array_equal(got_i, want_i)
     np.testing.assert_array_equal(got_d, want_d)
     del sorted_codes
@@ -230,3 +232,21 @@ def test_server_over_blocked_collection(gpu, big):
     (pd, pi), = plan_topk(torch, capi, [(sp, Qd, 37, codes, N_BIG)], K)
     np.testing.assert_array_equal(oi, pi)
     np.testing.assert_array_equal(od, pd)
+
+
+@pytest.mark.parametrize("k", [10, 100])
+def test_fp32_plan_large_k(gpu, big, k):
+    # The plain path's plans: fp32 distances in row blocks, then the row top-k.
+    import torch
+    capi = gpu
+    sp, codes, g = big
+    say(f"[gpu] fp32 plan k={k}")
+    m, n = 6, 300_000
+    Qd = torch.randn((m, 128), generator=g, device="cuda")
+    D = torch.empty((m, n), dtype=torch.float32, device="cuda")
+    sp.m_to_n(Qd, m, codes, n, D, mode=capi.TQ_MODE_FP32)
+    vals, idx = torch.sort(D, dim=1, stable=True)
+    want_d, want_i = vals[:, :k].cpu().numpy(), idx[:, :k].cpu().numpy()
+    (got_d, got_i), = plan_topk(torch, capi, [(sp, Qd, m, codes, n)], k, mode=capi.TQ_MODE_FP32)
+    np.testing.assert_array_equal(got_i, want_i)
+    np.testing.assert_array_equal(got_d, want_d)

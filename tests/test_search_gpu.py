@@ -303,7 +303,9 @@ def test_topk_rows_ties_by_column(gpu, torch_cuda, kind):
     say(f"[gpu] topk rows {kind}")
     rng = np.random.default_rng(5)
     for m, n, k in ((3, 100, 10), (20, 70000, 16), (1, 1, 1), (5, 4097, 7), (7, 8193, 16),
-                    (9, 13568, 10), (33, 2047, 16), (6, 2049, 3), (4, 24577, 16)):
+                    (9, 13568, 10), (33, 2047, 16), (6, 2049, 3), (4, 24577, 16),
+                    (5, 70000, 100), (3, 8193, 128), (4, 50, 100), (2, 24577, 65),
+                    (6, 3000, 17)):
         if kind == "ties":
             D = rng.integers(0, 50, (m, n)).astype(np.float32)  # many exact ties
         elif kind == "uniform":
@@ -327,9 +329,12 @@ def test_search_rejects_bad_arguments(gpu, torch_cuda):
     small = capi.Space(16, 4)  # padded dim 16: below the MFMA tile
     _, dQ = c.queries(torch, 2)
     od, oi = outputs(torch, 2, 16)
-    for k in (0, 17):
+    for k in (0, 129):
         with pytest.raises(capi.TQError, match="k must be"):
             capi.SearchPlan([(c.sp, dQ, 2, c.codes, 10, od, oi)], k)
+    od17, oi17 = outputs(torch, 2, 17)
+    with pytest.raises(capi.TQError, match="fused form takes k"):
+        capi.SearchPlan([(c.sp, dQ, 2, c.codes, 10, od17, oi17)], 17, form=capi.TQ_FORM_FUSED)
     with pytest.raises(capi.TQError, match="query_tile"):
         capi.SearchPlan([(c.sp, dQ, 2, c.codes, 10, od, oi)], 10, query_tile=48)
     with pytest.raises(capi.TQError, match="padded dim"):

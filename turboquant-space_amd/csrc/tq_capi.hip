@@ -115,64 +115,6 @@ void make_codebook(int mse_bits, std::vector<float> &cent, std::vector<float> &b
   }
 }
 
-// Centroid scale for the fp16 MFMA operand. Byte codes (>= 5 bits) keep
-// 2^8 and a second fp16 term (c = c_hi + c_lo). Nibble codes have at most 4
-// centroid magnitudes, so a scale alpha in [256, 512) (multiples of 2^-12,
-// exact in fp32) can be found for which fp16(fl32(c * alpha)) / alpha is
-// within 2^-15 of every c; then c_hi alone carries the centroid and the
-// q*c_lo contraction is dropped (its absence moves distances by at most
-// ~5e-5 of their scale, measured at D = 32..4096 against fp64). The
-// search mirrors the device arithmetic: fp32 product, fp16 round to nearest
-// even, fp32 reciprocal scale.
-double fp16_rne(double v) {  // v > 0 in fp16's normal range
-  int e = 0;
-  std::frexp(v, &e);  // v = f * 2^e, f in [0.5, 1)
-  const double ulp = std::ldexp(1.0, e - 11);
-  return std::nearbyint(v / ulp) * ulp;
-}
-
-struct CentScale {
-  float cscale;
-  float mse_unscale;
-  int lo;
-};
-
-CentScale centroid_scale(const std::vector<float> &cent, bool nib) {
-  if (nib) {
-    const size_t L = cent.size();
-    double best = 1.0, best_a = 256.0;
-    for (int k = 0; k < (1 << 20); ++k) {
-      const double a = 256.0 + std::ldexp((double)k, -12);
-      const double u = (double)(float)(1.0 / (1024.0 * a));
-      double worst = 0.0;
-      for (size_t l = L / 2; l < L && worst < best; ++l) {
-        const double c = std::fabs((double)cent[l]);
-        const double v = (double)(float)(c * a);
-        worst = std::max(worst, std::fabs(fp16_rne(v) * u * 1024.0 - c) / c);
-      }
-      if (worst < best) {
-        best = worst;
-        best_a = a;
-      }
-    }
-    if (best <= std::ldexp(1.0, -15))
-      return {(float)best_a, (float)(1.0 / (1024.0 * best_a)), 0};
-  }
-  return {256.0f, 1.0f / 262144.0f, 1};
-}
-
-// Cached per bit width (the codebook depends on nothing else).
-CentScale centroid_scale_cached(int bits, const std::vector<float> &cent) {
-  static std::mutex mu;
-  static std::map<int, CentScale> cache;
-  std::lock_guard<std::mutex> lk(mu);
-  auto it = cache.find(bits);
-  if (it != cache.end()) return it->second;
-  const CentScale cs = centroid_scale(cent, bits <= 4);
-  cache.emplace(bits, cs);
-  return cs;
-}
-
 // ---- scoped device buffer ------------------------------------------------
 
 struct DevBuf {
@@ -264,10 +206,10 @@ int tq_space_create(uint64_t dim, int bits, uint64_t rot_seed, uint64_t qjl_seed
   d.L = (int)L;
   d.scale = s->info.scale;
   d.inv_sqrt_D = 1.0f / std::sqrt(static_cast<float>(D));
-  const CentScale csc = centroid_scale_cached(bits, s->cent);
-  d.cscale = csc.cscale;
-  d.mse_unscale = csc.mse_unscale;
-  d.mfma_lo = csc.lo;
+  // Centroids times 2^8 stay within fp16's normal range together with their
+  // fp16 remainders (|c| < 4 at every bit width).
+  d.cscale = 256.0f;
+  d.mse_unscale = 1.0f / 262144.0f;
   d.cent = s->d_tables;
   d.bnd = s->d_tables + L;
   d.rot = s->d_tables + 2 * L;

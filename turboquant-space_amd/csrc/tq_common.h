@@ -23,11 +23,10 @@ struct SpaceDev {
   float scale;      // sqrt(pi/2)/sqrt(D), as the reference computes it in fp32
   float inv_sqrt_D; // 1.0f / sqrtf((float)D), correctly rounded
   // MFMA forms: the centroids enter the fp16 B operand as fp16(c * cscale)
-  // (+ fp16 remainder when mfma_lo); mse_unscale = fl(1 / (2^10 * cscale))
+  // plus the fp16 remainder; mse_unscale = 1 / (2^10 * cscale) (exact)
   // undoes the query and centroid scales in the epilogue.
   float cscale;
   float mse_unscale;
-  int mfma_lo;
   const float *cent;  // [L]   Lloyd-Max centroids (ascending)
   const float *bnd;   // [L-1] decision boundaries (ascending)
   const float *rot;   // [D]   +-1 rotation signs (rot_seed)

@@ -40,9 +40,9 @@ int plan_bind(tq_search_plan p, void *ws, size_t bytes, hipStream_t st);
 int plan_enqueue(tq_search_plan p, hipStream_t st);
 // MFMA plans can run a batch of `rows` queries over `n` codes blocked
 // (distance blocks + row top-k) when plan_blocked(n, rows), and do unless
-// their knobs ask for the fused form; plan_blocked_ws_bound(rows, n_max)
+// their knobs ask for the fused form; plan_blocked_ws_bound(rows, n_max, k)
 // bounds the workspace that adds for batches of up to `rows` queries over at
 // most n_max codes.
 bool plan_blocked(long n, long rows);
-size_t plan_blocked_ws_bound(long rows, long n_max);
+size_t plan_blocked_ws_bound(long rows, long n_max, int k);
 }  // namespace tq

@@ -1,6 +1,7 @@
 // tq_mfma_dev.h -- device helpers shared by the MFMA kernels (tq_mfma.hip,
-// tq_search.hip): fp16 hi/lo scaling constants and the code-word decoders
-// that turn packed code units into B-operand fragments.
+// tq_search.hip): the fp16 hi/lo scaling constants, the layout of the split
+// query planes, and the code-word decoders that turn packed code units into
+// B-operand fragments.
 #pragma once
 #include "tq_common.h"
 
@@ -14,6 +15,29 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 // (SpaceDev::cscale, undone with SpaceDev::mse_unscale).
 constexpr float kQScale = 1024.0f;              // 2^10 on q_rot and s_q
 constexpr float kQjlUnscale = 1.0f / 1024.0f;   // 2^-10
+
+// Split query planes. Plane 0 = q_hi, 1 = q_lo, 2 = s_hi, 3 = s_lo (fp16 of
+// the 2^10-scaled q_rot and s_q, lo = fp16 of the remainder). The asymmetric
+// form of a nibble space reads all four; byte spaces skip s_lo (their QJL
+// term carries a small gamma); the symmetric MSE form reads q_hi and q_lo.
+__host__ __device__ constexpr int split_planes(bool nib, bool qjl) {
+  return qjl ? (nib ? 4 : 3) : 2;
+}
+// Layout: blocks of one MFMA A fragment (32 queries x 16 K slots, 1 KB),
+// [query block rb][k step ks][plane p][lane l][8 halves], ks = 0 .. D/16 - 1.
+// Lane l = lr + 32 kg of block (rb, ks) holds query 32 rb + lr; its element
+// e is coordinate 32 (ks / 2) + 16 kg + 8 (ks % 2) + perm(e), where perm is
+// (0, 2, 4, 6, 1, 3, 5, 7) for nibble spaces (the order decode_nib_word emits
+// a code word's coordinates in) and the identity for byte spaces. So within a
+// 32-coordinate chunk, lane half kg covers code bytes [8 kg, 8 kg + 8)
+// (nibbles) or [16 kg, 16 kg + 16) (bytes) for both k steps.
+constexpr int kBlockHalves = 512;
+__host__ __device__ inline long split_planes_halves(int D, long m, int npl) {
+  return (m + 31) / 32 * (long)(D / 16) * npl * kBlockHalves;
+}
+__host__ __device__ inline int split_perm(bool nib, int e) {
+  return nib ? (e < 4 ? 2 * e : 2 * (e - 4) + 1) : e;
+}
 
 // fp16 hi and lo parts of the scaled centroid k (0 past the L centroids).
 __device__ __forceinline__ void cent_h(const SpaceDev &s, int k, _Float16 &ch, _Float16 &cl) {
@@ -35,11 +59,27 @@ struct PermTab {
   uint32_t lh_lo, lh_hi;  // high bytes of c_lo
 };
 
-// 8 nibble units in one code word -> B fragments in the order
-// (0, 2, 4, 6, 1, 3, 5, 7): bytes of the even (odd) coordinates' values are
-// looked up four at a time, then interleaved into fp16 pairs. The c_lo
-// fragment is decoded only for spaces that need it (SpaceDev::mfma_lo).
-template <bool LO>
+// The perm tables of a nibble space's 8 centroids.
+__device__ __forceinline__ PermTab perm_tab(const SpaceDev &s) {
+  uint32_t bytes[4][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    _Float16 chh, cll;
+    cent_h(s, k, chh, cll);
+    const uint32_t hb = h_bits(chh), lb = h_bits(cll);
+    const int word = k >> 2, sh = 8 * (k & 3);
+    bytes[0][word] |= (hb & 0xFFu) << sh;
+    bytes[1][word] |= (hb >> 8) << sh;
+    bytes[2][word] |= (lb & 0xFFu) << sh;
+    bytes[3][word] |= (lb >> 8) << sh;
+  }
+  return PermTab{bytes[0][0], bytes[0][1], bytes[1][0], bytes[1][1],
+                 bytes[2][0], bytes[2][1], bytes[3][0], bytes[3][1]};
+}
+
+// 8 nibble units in one code word -> B fragments (c_hi, c_lo, +-1) in the
+// order (0, 2, 4, 6, 1, 3, 5, 7): bytes of the even (odd) coordinates' values
+// are looked up four at a time, then interleaved into fp16 pairs.
 __device__ __forceinline__ void decode_nib_word(uint32_t w, const PermTab &t, h8 &bh, h8 &bl,
                                                 h8 &bt) {
   const uint32_t ie = (w >> 1) & 0x07070707u;  // indices of coordinates 0, 2, 4, 6
@@ -56,7 +96,7 @@ __device__ __forceinline__ void decode_nib_word(uint32_t w, const PermTab &t, h8
     r[3] = __builtin_amdgcn_perm(Ho, Lo, 0x07030602u);
     bh = __builtin_bit_cast(h8, r);
   }
-  if constexpr (LO) {
+  {
     const uint32_t Le = __builtin_amdgcn_perm(t.ll_hi, t.ll_lo, ie);
     const uint32_t He = __builtin_amdgcn_perm(t.lh_hi, t.lh_lo, ie);
     const uint32_t Lo = __builtin_amdgcn_perm(t.ll_hi, t.ll_lo, io);

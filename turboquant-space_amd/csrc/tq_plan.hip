@@ -242,7 +242,10 @@ int layout(tq_search_plan_s *p) {
   for (PlanBatch &b : p->batches) {
     const tq_search_tuning t = knobs_for(p, b);
     b.qr = t.query_tile == 64 ? 2 : 1;
-    b.blocked = p->mode == TQ_MODE_MFMA && t.form == TQ_FORM_BLOCKED &&
+    // k above the fused form's list length always runs blocked (plan_new
+    // refuses plans where that is not possible).
+    b.blocked = p->mode == TQ_MODE_MFMA &&
+                (t.form == TQ_FORM_BLOCKED || p->k > tq::kSearchMaxK) &&
                 b.space->info.padded_dim >= 32 && tq::plan_blocked(b.n, b.m);
     const double cost = (double)b.m * (double)b.n * (double)b.space->info.padded_dim;
     if (cost > worst) {
@@ -342,21 +345,21 @@ int layout(tq_search_plan_s *p) {
     p->o_q[bi] = take((size_t)b.m * info.dim * sizeof(float));
     p->o_prep[bi] = take((size_t)b.m * info.prepared_stride * sizeof(float));
     if (p->mode == TQ_MODE_MFMA) {
-      p->o_planes[bi] = take(3 * (size_t)b.m * info.padded_dim * sizeof(_Float16));
+      p->o_planes[bi] = take(tq::split_planes_bytes(b.space->dev, (long)b.m));
       if (b.blocked) {
         long nc = (long)(tq::kMfmaBlockBytes / (sizeof(float) * (size_t)b.m));
         nc = std::max(4096L, nc - nc % tq::kSearchNTile);
         b.nc = std::min(nc, b.n);
         b.lists = 1 + tq::topk_rows_segments(b.m, b.nc);
         dist_elems = std::max(dist_elems, (size_t)b.m * b.nc);
-        seg_elems = std::max(seg_elems, (size_t)b.m * b.lists * tq::kSearchMaxK);
+        seg_elems = std::max(seg_elems, (size_t)b.m * b.lists * p->k);
       }
     } else if (b.m > 0) {
       const long rows = fp32_block_rows(b);
       dist_elems = std::max(dist_elems, (size_t)rows * b.n);
       // A block of r <= rows rows uses r * topk_rows_segments(r, n) lists.
       seg_elems = std::max(seg_elems,
-                           (size_t)rows * tq::topk_rows_segments(rows, b.n) * tq::kSearchMaxK);
+                           (size_t)rows * tq::topk_rows_segments(rows, b.n) * p->k);
       scan_ws = std::max(scan_ws, tq::scan_fp32_ws_bytes(b.space->dev, rows));
     }
   }
@@ -489,10 +492,13 @@ int enqueue(tq_search_plan_s *p, hipStream_t st) {
       return hip_fail(e, what);
   }
   if (p->mode == TQ_MODE_MFMA) {
-    e = tq::launch_search(p->d_jobs, p->d_items, p->d_order, p->nitems_qr, p->d_groups,
-                          (int)p->h_groups.size(), p->k, p->d_counter, p->cand_d, p->cand_i,
-                          p->grid_qr, st);
-    if (e != hipSuccess) return hip_fail(e, what);
+    // Fused batches (none when every batch runs blocked, as for k > 16).
+    if (!p->h_items.empty() || !p->h_groups.empty()) {
+      e = tq::launch_search(p->d_jobs, p->d_items, p->d_order, p->nitems_qr, p->d_groups,
+                            (int)p->h_groups.size(), p->k, p->d_counter, p->cand_d, p->cand_i,
+                            p->grid_qr, st);
+      if (e != hipSuccess) return hip_fail(e, what);
+    }
     // Blocked batches: per chunk of codes, the distance block, its row top-k
     // as lists 1.. of each query, folded into list 0; the last fold writes
     // the query's k best.
@@ -586,6 +592,16 @@ int plan_new(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
       return fail(TQ_EUNSUPPORTED, "%s: job %llu: MFMA search needs padded dim >= 32, got %d", what,
                   (unsigned long long)i, j.space->dev.D);
   }
+  // k above the fused form's per-lane lists runs blocked only.
+  if (mode == TQ_MODE_MFMA && k > tq::kSearchMaxK) {
+    if (tuning && tuning->form == TQ_FORM_FUSED)
+      return fail(TQ_EINVAL, "%s: the fused form takes k <= %d, got %d", what, tq::kSearchMaxK, k);
+    uint64_t rows = 0;
+    for (uint64_t i = 0; i < njobs; ++i) rows += jobs[i].m;
+    if (rows > 65535)
+      return fail(TQ_EINVAL, "%s: k > %d runs blocked, which takes at most 65535 queries per plan",
+                  what, tq::kSearchMaxK);
+  }
   tq_search_plan_s *p = new (std::nothrow) tq_search_plan_s();
   if (!p) return fail(TQ_ENOMEM, "%s: host allocation failed", what);
   p->device = device;
@@ -642,7 +658,7 @@ bool plan_blocked(long n, long rows) {
   return rows > 0 && rows <= 65535 && n > 0 && n < 0xFFFFFFFFL;
 }
 
-size_t plan_blocked_ws_bound(long rows, long n_max) {
+size_t plan_blocked_ws_bound(long rows, long n_max, int k) {
   // The distance block: rows x nc floats, nc <= min(n, max(4096,
   // kMfmaBlockBytes / (4 * rows))); and (1 + segments) lists per query over a
   // chunk, segments = topk_rows_segments(rows, nc) = ceil(nc / 8192):
@@ -653,7 +669,7 @@ size_t plan_blocked_ws_bound(long rows, long n_max) {
   const size_t block = std::min(std::max(kMfmaBlockBytes, r * 4096 * sizeof(float)),
                                 r * (size_t)std::max(n_max, 1L) * sizeof(float));
   const size_t lists = 4 * r + kMfmaBlockBytes / 8192;
-  return align_up(block) + 2 * align_up(lists * kSearchMaxK * sizeof(float)) + 4096;
+  return align_up(block) + 2 * align_up(lists * (size_t)std::max(k, 1) * sizeof(float)) + 4096;
 }
 
 }  // namespace tq
@@ -738,8 +754,9 @@ int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen
       float best_ms = 0.0f, best_form_ms = 0.0f;
       int best_form = TQ_FORM_FUSED;
       std::vector<tq_search_tuning> cands;
-      for (int qt : {32, 64})
-        for (int ipw : {1, 2, 4, 8}) cands.push_back(tq_search_tuning{qt, ipw, TQ_FORM_FUSED});
+      if (p->k <= tq::kSearchMaxK)
+        for (int qt : {32, 64})
+          for (int ipw : {1, 2, 4, 8}) cands.push_back(tq_search_tuning{qt, ipw, TQ_FORM_FUSED});
       if (b.space->info.padded_dim >= 32 && tq::plan_blocked(b.n, b.m))
         cands.push_back(tq_search_tuning{0, 0, TQ_FORM_BLOCKED});
       for (size_t ci = 0; ci < cands.size(); ++ci) {
@@ -828,7 +845,7 @@ int tq_topk_rows(const float *dist, uint64_t m, uint64_t n, int k, float *out_di
   if (m == 0) return TQ_OK;
   if (!out_dist || !out_ids || (n > 0 && !dist)) return fail(TQ_EINVAL, "%s: NULL buffer", what);
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const size_t seg = (size_t)m * tq::topk_rows_segments((long)m, (long)n) * tq::kSearchMaxK;
+  const size_t seg = (size_t)m * tq::topk_rows_segments((long)m, (long)n) * (size_t)k;
   void *buf = nullptr;
   hipError_t e = hipMallocAsync(&buf, tq::align_up(seg * sizeof(float)) + seg * sizeof(uint32_t), st);
   if (e != hipSuccess) return hip_fail(e, what);

@@ -36,8 +36,7 @@ using namespace mfma;
 
 constexpr int kSN = kSearchNTile;  // codes per tile (64 per wavefront)
 constexpr int kSKC = 32;           // coordinates per chunk
-constexpr int kSARow = kSKC + 8;   // padded query-plane row, in halves
-constexpr int kSPlanes = 3;        // q_hi, q_lo, s_hi
+constexpr int kSPlanes = 4;        // q_hi, q_lo, s_hi, s_lo (byte spaces read three)
 constexpr int kSCRow = 36;         // padded code row in bytes (8-bit chunks need 32)
 constexpr int kTRow = 65;          // padded transpose row, in floats
 
@@ -113,11 +112,12 @@ template <int QR>
 struct SearchShape {
   static constexpr int QT = 32 * QR;  // queries per item
   static constexpr int WPE = 3 - QR;  // waves per SIMD the registers allow
-  static constexpr int STAGE = 2 * kSPlanes * QT * kSARow * 2 + 2 * kSN * kSCRow;
+  static constexpr int NBMAX = kSPlanes * QR * 2;  // 1 KB query-plane blocks per chunk
+  static constexpr int ABYTES = NBMAX * 1024;
+  static constexpr int STAGE = 2 * ABYTES + 2 * kSN * kSCRow;
   static constexpr int TRANS = 4 * 32 * kTRow * 4;
   static constexpr int LDS = STAGE > TRANS ? STAGE : TRANS;
-  static constexpr int AUNITS = kSPlanes * QT * (kSKC / 8);  // 16-byte query-plane loads per chunk
-  static constexpr int ALOADS = (AUNITS + 255) / 256;        // per thread
+  static constexpr int ALOADS = NBMAX * 64 / 256;  // 16-byte query-plane loads per thread
 };
 
 template <int QR, int K>
@@ -134,8 +134,8 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
   __shared__ uint32_t l_tab[256];
   __shared__ float l_qn[QT][2];
   __shared__ int s_item;
-  auto a_lds = reinterpret_cast<_Float16(*)[kSPlanes][QT][kSARow]>(lds);
-  auto c_lds = reinterpret_cast<uint32_t(*)[kSN * kSCRow / 4]>(lds + 2 * kSPlanes * QT * kSARow * 2);
+  auto a_lds = reinterpret_cast<_Float16(*)[Sh::ABYTES / 2]>(lds);
+  auto c_lds = reinterpret_cast<uint32_t(*)[kSN * kSCRow / 4]>(lds + 2 * Sh::ABYTES);
   auto t_lds = reinterpret_cast<float(*)[32][kTRow]>(lds);
 
   const int tid = threadIdx.x;
@@ -156,13 +156,12 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
     const bool nib = s.nib != 0;
     const long m = job.m, n = job.n;
     const long q0 = (long)item.qtile * QT;
-    const long mD = m * (long)s.D;
     const int nchunks = s.D / kSKC;
+    const int nks = s.D / 16;
     const int cparts = nib ? 1 : 2;  // 16-byte loads per code and chunk
+    const int npl = split_planes(nib, true);
+    const int nb = npl * QR * 2;    // query-plane blocks per chunk
 
-    // c_lo pass only where the space's scaled centroids are not exact in
-    // fp16 (all byte codes; nibble codes normally not).
-    const bool lo = !nib || s.mfma_lo != 0;
     for (int i = tid; i < 256; i += blockDim.x) {
       _Float16 ch, cl;
       cent_h(s, i, ch, cl);
@@ -175,39 +174,26 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
       l_qn[tid][1] = pq[2 * s.D + 1];
     }
     PermTab ptab{};
-    if (nib) {
-      uint32_t bytes[4][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}, {0u, 0u}};
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        _Float16 chh, cll;
-        cent_h(s, k, chh, cll);
-        const uint32_t hb = h_bits(chh), lb = h_bits(cll);
-        const int word = k >> 2, sh = 8 * (k & 3);
-        bytes[0][word] |= (hb & 0xFFu) << sh;
-        bytes[1][word] |= (hb >> 8) << sh;
-        bytes[2][word] |= (lb & 0xFFu) << sh;
-        bytes[3][word] |= (lb >> 8) << sh;
-      }
-      ptab = PermTab{bytes[0][0], bytes[0][1], bytes[1][0], bytes[1][1],
-                     bytes[2][0], bytes[2][1], bytes[3][0], bytes[3][1]};
-    }
+    if (nib) ptab = perm_tab(s);
 
-    // Query-plane loads: 3 planes x QT rows x 32 fp16, 16 B each.
+    // Query-plane loads: the chunk's nb blocks of 1 KB (tq_mfma_dev.h),
+    // block i = (row block r, k step kb, plane p), i = (r * 2 + kb) * npl + p,
+    // 16 bytes per thread and load. Query blocks past m read the last block.
+    const long nrb = (m + 31) / 32;
     const _Float16 *asrc[ALOADS];
-    bool aok[ALOADS], ain[ALOADS];
+    bool ain[ALOADS];
     int adst[ALOADS];
 #pragma unroll
     for (int i = 0; i < ALOADS; ++i) {
       const int idx0 = i * 256 + tid;
-      ain[i] = idx0 < Sh::AUNITS;  // with 32-query tiles the last round is half used
+      ain[i] = idx0 < nb * 64;
       const int idx = ain[i] ? idx0 : 0;
-      const int p = idx / (QT * kSKC / 8);
-      const int row = (idx / (kSKC / 8)) % QT;
-      const int part = idx % (kSKC / 8);
-      const long q = q0 + row;
-      aok[i] = ain[i] && q < m;
-      asrc[i] = job.planes + p * mD + (aok[i] ? q : 0L) * s.D + 8 * part;
-      adst[i] = (p * QT + row) * kSARow + 8 * part;
+      const int blk = idx >> 6, l = idx & 63;
+      const int r = blk / (2 * npl), kb = (blk / npl) & 1, p = blk % npl;
+      long rb = q0 / 32 + r;
+      if (rb >= nrb) rb = nrb - 1;
+      asrc[i] = job.planes + ((rb * nks + kb) * npl + p) * (long)kBlockHalves + 8 * l;
+      adst[i] = idx * 8;
     }
 
     float td[K];
@@ -235,7 +221,8 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
         const int k0 = ch * kSKC;
 #pragma unroll
         for (int i = 0; i < ALOADS; ++i)
-          ra[i] = aok[i] ? *reinterpret_cast<const uint4 *>(asrc[i] + k0) : make_uint4(0u, 0u, 0u, 0u);
+          ra[i] = ain[i] ? *reinterpret_cast<const uint4 *>(asrc[i] + (long)ch * 2 * npl * kBlockHalves)
+                         : make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
           if (cok[i]) {
@@ -247,7 +234,7 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
         }
       };
       auto store_chunk = [&](int b) {
-        _Float16 *abuf = &a_lds[b][0][0][0];
+        _Float16 *abuf = &a_lds[b][0];
 #pragma unroll
         for (int i = 0; i < ALOADS; ++i)
           if (ain[i]) *reinterpret_cast<uint4 *>(abuf + adst[i]) = ra[i];
@@ -263,11 +250,13 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
         }
       };
 
-      // The tile's contraction and its distances into the lists, with the
-      // c_lo pass compiled in or out (a branch around MFMAs inside the
-      // unrolled loop would split its schedule).
-      auto scan_tile = [&](auto lo_c) __attribute__((always_inline)) {
-        constexpr bool LO = decltype(lo_c)::value;
+      // The tile's contraction and its distances into the lists: the passes
+      // and their order are k_mfma's (the s_lo pass for nibble spaces only),
+      // compiled per kind (a branch around MFMAs inside the unrolled loop
+      // would split its schedule).
+      auto scan_tile = [&](auto nib_c) __attribute__((always_inline)) {
+        constexpr bool NIB = decltype(nib_c)::value;
+        constexpr int NPL = split_planes(NIB, true);
         f16v acc_m[QR][2], acc_q[QR][2];
 #pragma unroll
         for (int r = 0; r < QR; ++r)
@@ -288,22 +277,23 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
           if (ch + 1 < nchunks) load_chunk(ch + 1);
 #pragma unroll
           for (int kb = 0; kb < kSKC / 16; ++kb) {
-            h8 a[QR][kSPlanes];
+            h8 a[QR][NPL];
 #pragma unroll
             for (int r = 0; r < QR; ++r)
 #pragma unroll
-              for (int p = 0; p < kSPlanes; ++p)
-                a[r][p] = *reinterpret_cast<const h8 *>(&a_lds[b][p][32 * r + lr][kb * 16 + 8 * kg]);
+              for (int p = 0; p < NPL; ++p)
+                a[r][p] = *reinterpret_cast<const h8 *>(
+                    &a_lds[b][((r * 2 + kb) * NPL + p) * kBlockHalves + 8 * lane]);
             h8 bh[2], bl[2], bt[2];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
               const int code = wave * 64 + 32 * c + lr;
-              if (nib) {
-                const uint32_t w = c_lds[b][(code * kSCRow + kb * 8 + 4 * kg) / 4];
-                decode_nib_word<LO>(w, ptab, bh[c], bl[c], bt[c]);
+              if constexpr (NIB) {
+                const uint32_t w = c_lds[b][(code * kSCRow + 8 * kg + 4 * kb) / 4];
+                decode_nib_word(w, ptab, bh[c], bl[c], bt[c]);
               } else {
-                const uint32_t w0 = c_lds[b][(code * kSCRow + kb * 16 + 8 * kg) / 4];
-                const uint32_t w1 = c_lds[b][(code * kSCRow + kb * 16 + 8 * kg) / 4 + 1];
+                const uint32_t w0 = c_lds[b][(code * kSCRow + 16 * kg + 8 * kb) / 4];
+                const uint32_t w1 = c_lds[b][(code * kSCRow + 16 * kg + 8 * kb) / 4 + 1];
                 uint32_t u[8];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
@@ -320,9 +310,8 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
             TQ_PASS(acc_m, 0, bh)
             TQ_PASS(acc_q, 2, bt)
             TQ_PASS(acc_m, 1, bh)
-            if constexpr (LO) {
-              TQ_PASS(acc_m, 0, bl)
-            }
+            if constexpr (NPL > 3) { TQ_PASS(acc_q, 3, bt) }
+            TQ_PASS(acc_m, 0, bl)
 #undef TQ_PASS
           }
           if (ch + 1 < nchunks) store_chunk(b ^ 1);
@@ -376,7 +365,7 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
           __syncthreads();
         }
       };
-      if (lo)
+      if (nib)
         scan_tile(std::true_type{});
       else
         scan_tile(std::false_type{});
@@ -469,26 +458,114 @@ __global__ __launch_bounds__(256) void k_topk_merge(const SearchJobDev *__restri
   wave_pop_topk<K>(td, ti, k_out, lane, job.out_d + q * k_out, job.out_i + q * k_out);
 }
 
-// `rounds` (<= K, wavefront-uniform) pops of the wavefront's best entries
-// from lane-held sorted lists of L entries; lane 0 stores them at (out_d,
-// out_i) as a sorted list, and the list's remaining K - rounds entries are
-// set empty (+inf, ~0).
-template <int L, int K>
-__device__ __forceinline__ void wave_pop_list(float (&td)[L], uint32_t (&ti)[L], float *out_d,
-                                              uint32_t *out_i, int lane, int rounds = K) {
-  if (lane >= rounds && lane < K) {
-    out_d[lane] = __builtin_inff();
-    out_i[lane] = 0xFFFFFFFFu;
+// Row-wise top-k of dense distance blocks (the blocked search form and the
+// plain fp32 plans), for k up to kTopkMaxK. Lists are sorted (distance, id)
+// entries, `lcap` per list (lcap = the k asked for), empty entries (+inf, ~0)
+// at the tail.
+//
+// Both stages are one wavefront selecting the kk smallest (distance, id)
+// entries of a set (a segment of a row's columns, or a row's lists):
+//   1. A threshold T from a sample (the set's first 64 x kTopkPL entries,
+//      kTopkPL per lane in registers): with r = 1 (kk <= 64) or 2, each lane's
+//      r-th smallest sample entry, and T = the ceil(kk / r)-th smallest of
+//      those 64 values. ceil(kk / r) lanes then hold r entries each at or below
+//      T, so at least kk entries of the set lie at or below T and the set's kk
+//      best all do. A caller's bound (the row's running kk-th entry) can lower
+//      T further: entries above it cannot be among the row's kk best.
+//   2. The real entries at or below T are packed into a per-wavefront LDS list
+//      by ballot and lane prefix counts, in set order.
+//   3. If more than kTopkCap were packed, T becomes the kk-th smallest of the
+//      first kTopkCap packed entries (all at or below T, so T only drops and
+//      at least kk entries stay at or below it) and the set is packed again,
+//      until the list holds them all (exact either way; typical rows never
+//      come here, adversarial orders a few times).
+//   4. Lanes take the packed entries (up to 4 or 16 each) into sorted
+//      lists, and min(count, kk) wavefront-minimum pops give the result in
+//      order; result r stays with lane r % 64 (slot r / 64) for one
+//      coalesced store.
+// Only the wavefront's own lanes touch its LDS list: no barrier.
+constexpr int kTopkSub = 2048;             // sample / pass: 64 lanes x kTopkPL
+constexpr int kTopkPL = kTopkSub / 64;
+constexpr int kBufferWord3 = 0x00020000;  // raw 32-bit buffer resource (gfx9 DATA_FORMAT)
+constexpr int kTopkSeg = 4 * kTopkSub;     // columns per (row, segment) of stage 1
+constexpr int kTopkCap = 1024;             // packed entries per wavefront
+
+// kq (1..64, wavefront-uniform) pops of the 64 lanes' values: the kq-th
+// smallest, returned to every lane.
+__device__ __forceinline__ void wave_kth(float d, uint32_t i, int kq, float &td, uint32_t &ti) {
+  td = __builtin_inff();
+  ti = 0xFFFFFFFFu;
+  for (int r = 0; r < kq; ++r) {
+    float bd = d;
+    uint32_t bi = i;
+    wave_lex_min(bd, bi);
+    td = bd;
+    ti = bi;
+    if (d == bd && i == bi) {  // ids are unique; empty entries popping together is harmless
+      d = __builtin_inff();
+      i = 0xFFFFFFFFu;
+    }
   }
-  for (int k = 0; k < rounds; ++k) {
+}
+
+// A lane's r-th smallest (r = 1 or 2) of its entries (d[u], id[u]).
+struct LaneRth {
+  float d0 = __builtin_inff(), d1 = __builtin_inff();
+  uint32_t i0 = 0xFFFFFFFFu, i1 = 0xFFFFFFFFu;
+  __device__ __forceinline__ void add(float d, uint32_t i) {
+    if (lex_less(d, i, d1, i1)) {
+      if (lex_less(d, i, d0, i0)) {
+        d1 = d0;
+        i1 = i0;
+        d0 = d;
+        i0 = i;
+      } else {
+        d1 = d;
+        i1 = i;
+      }
+    }
+  }
+};
+
+// Step 4: the min(n, kk) smallest of the n <= kTopkCap packed entries, in
+// order; result r into lane r % 64, slot r / 64 (empty past the last).
+// Returns the last result (wavefront-uniform).
+template <int L>
+__device__ __forceinline__ void topk_pop_packed(const float *cd, const uint32_t *ci, int n, int kk,
+                                                int lane, float (&rd)[2], uint32_t (&ri)[2],
+                                                float &last_d, uint32_t &last_i) {
+  float td[L];
+  uint32_t ti[L];
+  topk_clear<L>(td, ti);
+  const int per = (n + 63) / 64;
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    if (t < per) {  // wavefront-uniform
+      const int q = lane + 64 * t;
+      if (q < n) topk_insert<L>(td, ti, cd[q], ci[q]);
+    }
+  }
+  rd[0] = rd[1] = __builtin_inff();
+  ri[0] = ri[1] = 0xFFFFFFFFu;
+  last_d = __builtin_inff();
+  last_i = 0xFFFFFFFFu;
+  const int np = n < kk ? n : kk;
+  for (int r = 0; r < np; ++r) {
     float bd = td[0];
     uint32_t bi = ti[0];
     wave_lex_min(bd, bi);
-    if (lane == 0) {
-      out_d[k] = bd;
-      out_i[k] = bi;
+    if ((r & 63) == lane) {
+      if (r < 64) {
+        rd[0] = bd;
+        ri[0] = bi;
+      } else {
+        rd[1] = bd;
+        ri[1] = bi;
+      }
     }
-    if (td[0] == bd && ti[0] == bi) {  // ids are unique; empty entries popping together is harmless
+    last_d = bd;
+    last_i = bi;
+    if (td[0] == bd && ti[0] == bi) {
 #pragma unroll
       for (int t = 0; t + 1 < L; ++t) {
         td[t] = td[t + 1];
@@ -500,288 +577,226 @@ __device__ __forceinline__ void wave_pop_list(float (&td)[L], uint32_t (&ti)[L],
   }
 }
 
-// Row-wise top-k of a dense distance matrix, stage 1: the m x nseg
-// (row, segment) pairs of kTopkSeg columns, row-major, go to the wavefronts
-// in turn (wavefront w of workgroup x takes pairs 4x + w, + 4 * gridDim.x,
-// ...; rows shorter than four segments share a workgroup), each segment read
-// as sub-blocks of kTopkSub columns (32 per lane, in registers); a segment's
-// best kk (<= K; the rest of the list empty) go to list seg0 + seg of its
-// row's `stride` lists of K entries (ids = column + id0). Only a row's kk
-// best are ever asked for, and they are among its segments' kk best.
-//   1. Every lane's minimum over the first sub-block; thr = the kk-th
-//      smallest of the 64 minima. They are kk distinct entries of the
-//      segment, so the segment's kk best all lie at or below thr. With
-//      use_run, list 0 of the row holds the row's kk best over the columns
-//      before this block (ids below every id here), and its kk-th entry is
-//      thr instead when it is real: nothing above it can be among the row's
-//      kk best, so the segment's list keeps only entries at or below it
-//      (fewer than kk when few are).
-//   2. The entries at or below thr (on typical rows a few dozen per
-//      segment) are packed into a per-wavefront LDS list by ballot and lane
-//      prefix counts, sub-block by sub-block.
-//   3. Lanes take them four apiece, sort their four, and min(count, kk)
-//      wavefront-minimum pops give the sorted list. If more than kTopkCap entries lie at or
-//      below thr, the segment is read again and each lane sorts its entries
-//      at or below thr into a list of K (exact either way).
-// Loads are unconditional (column index clamped) so a sub-block's loads are
-// all in flight together. Only the wavefront's own lanes touch its LDS list:
-// no barrier.
-constexpr int kTopkSub = 2048;
-constexpr int kBufferWord3 = 0x00020000;  // raw 32-bit buffer resource (gfx9 DATA_FORMAT)
-constexpr int kTopkSeg = 4 * kTopkSub;
-constexpr int kTopkCap = 256;
-template <int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_topk_rows(
-    const float *__restrict__ dist, long m, long n,
-                                                   long nseg, long stride, long seg0, uint32_t id0,
-                                                   int kk, int use_run, float *__restrict__ seg_d,
-                                                   uint32_t *__restrict__ seg_i) {
-  constexpr int PL = kTopkSub / 64;  // columns per lane and sub-block
-  __shared__ float c_d[4][kTopkCap];
-  __shared__ uint32_t c_i[4][kTopkCap];
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  // Within a segment, columns are 32-bit offsets jl < nl from the segment's
-  // first column, read through a buffer resource of the segment (a 32-bit
-  // byte offset per lane, no 64-bit addresses in registers). Every offset is
-  // clamped to the segment's last column in the vector offset itself, so no
-  // load leaves the segment whatever the hardware's range check covers;
-  // columns past nl are then set to +inf.
-  const float *ps = dist;
-  int nl = 0;
-  auto load_sub = [&](float (&v)[PL], __amdgpu_buffer_rsrc_t rs, int sb)
-                      __attribute__((always_inline)) {
+__device__ __forceinline__ void topk_pop_any(const float *cd, const uint32_t *ci, int n, int kk,
+                                             int lane, float (&rd)[2], uint32_t (&ri)[2],
+                                             float &last_d, uint32_t &last_i) {
+  if (n <= 4 * 64)
+    topk_pop_packed<4>(cd, ci, n, kk, lane, rd, ri, last_d, last_i);
+  else
+    topk_pop_packed<16>(cd, ci, n, kk, lane, rd, ri, last_d, last_i);
+}
+
+// Steps 1-4 over a set of E entries that SRC loads kTopkPL per lane per
+// pass: src.pass(p, d, id) fills entries e = p * kTopkSub + u * 64 + lane
+// (empty past E). (hint_d, hint_i) bounds T from above when real.
+template <class SRC>
+__device__ __forceinline__ void wave_select(const SRC &src, long E, int kk, float hint_d,
+                                            uint32_t hint_i, float *cd, uint32_t *ci, int lane,
+                                            float (&rd)[2], uint32_t (&ri)[2], int &np) {
+  float v[kTopkPL];
+  uint32_t vi[kTopkPL];
+  src.pass(0, v, vi);
+  // 1. T from the sample.
+  const int r = kk > 64 ? 2 : 1;
+  const int kq = (kk + r - 1) / r;
+  LaneRth lr;
 #pragma unroll
-    for (int u = 0; u < PL; ++u) {
-      const int jl = sb * kTopkSub + u * 64 + lane;
-      const int jc = jl < nl ? jl : nl - 1;
-      v[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, 4 * jc, 0, 0));
-    }
+  for (int u = 0; u < kTopkPL; ++u) lr.add(v[u], vi[u]);
+  float T_d;
+  uint32_t T_i;
+  wave_kth(r == 1 ? lr.d0 : lr.d1, r == 1 ? lr.i0 : lr.i1, kq, T_d, T_i);
+  if (hint_i != 0xFFFFFFFFu && lex_less(hint_d, hint_i, T_d, T_i)) {
+    T_d = hint_d;
+    T_i = hint_i;
+  }
+  const long npass = (E + kTopkSub - 1) / kTopkSub;
+  // 2. pack the real entries at or below T.
+  auto pack = [&](bool reload0) -> long {
+    long cnt = 0;
+    for (long p = 0; p < npass; ++p) {
+      if (p > 0 || reload0) src.pass(p, v, vi);
 #pragma unroll
-    for (int u = 0; u < PL; ++u)
-      if (sb * kTopkSub + u * 64 + lane >= nl) v[u] = __builtin_inff();
-  };
-  const long pairs = m * nseg;
-  for (long t = (long)blockIdx.x * 4 + w; t < pairs; t += (long)gridDim.x * 4) {
-    const long row = t / nseg, seg = t - row * nseg;
-    float *od = seg_d + (row * stride + seg0 + seg) * K;
-    uint32_t *oi = seg_i + (row * stride + seg0 + seg) * K;
-    if (n <= 0) {  // no columns (dist may be null): an empty list
-      if (lane < K) {
-        od[lane] = __builtin_inff();
-        oi[lane] = 0xFFFFFFFFu;
-      }
-      continue;
-    }
-    const long c0 = seg * kTopkSeg;
-    ps = dist + row * n + c0;
-    nl = (int)(n - c0 < kTopkSeg ? n - c0 : kTopkSeg);  // >= 1
-    const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(ps), 0, 4 * nl, kBufferWord3);
-    const uint32_t idb = (uint32_t)c0 + id0;            // id of column jl: idb + jl
-    float v[PL];
-    load_sub(v, rs, 0);
-    // 1. thr: the row's running kk-th entry, else from the lane minima over
-    // the first sub-block.
-    float thr_d = __builtin_inff();
-    uint32_t thr_i = 0xFFFFFFFFu;
-    if (use_run) {
-      thr_d = seg_d[row * stride * K + kk - 1];
-      thr_i = seg_i[row * stride * K + kk - 1];
-    }
-    if (thr_i == 0xFFFFFFFFu) {  // wavefront-uniform
-      float md = __builtin_inff();
-      uint32_t mi = 0xFFFFFFFFu;
-#pragma unroll
-      for (int u = 0; u < PL; ++u) {
-        const int jl = u * 64 + lane;
-        const uint32_t id = jl < nl ? idb + (uint32_t)jl : 0xFFFFFFFFu;
-        if (lex_less(v[u], id, md, mi)) {
-          md = v[u];
-          mi = id;
-        }
-      }
-      for (int r = 0; r < kk; ++r) {
-        float bd = md;
-        uint32_t bi = mi;
-        wave_lex_min(bd, bi);
-        thr_d = bd;
-        thr_i = bi;
-        if (md == bd && mi == bi) {
-          md = __builtin_inff();
-          mi = 0xFFFFFFFFu;
-        }
-      }
-    }
-    // 2. pack the entries at or below thr, sub-block by sub-block.
-    int cnt = 0;
-    for (int sb = 0; sb < kTopkSeg / kTopkSub; ++sb) {
-      if (sb * kTopkSub >= nl) break;  // wavefront-uniform
-      if (sb > 0) load_sub(v, rs, sb);
-#pragma unroll
-      for (int u = 0; u < PL; ++u) {
-        const int jl = sb * kTopkSub + u * 64 + lane;
-        const uint32_t id = idb + (uint32_t)jl;
-        const bool cand = jl < nl && !lex_less(thr_d, thr_i, v[u], id);
+      for (int u = 0; u < kTopkPL; ++u) {
+        const bool cand = vi[u] != 0xFFFFFFFFu && !lex_less(T_d, T_i, v[u], vi[u]);
         const uint64_t b = __ballot(cand);
-        const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi(
-                                  (uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+        const long pos = cnt + (long)__builtin_amdgcn_mbcnt_hi(
+                                   (uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
         if (cand && pos < kTopkCap) {
-          c_d[w][pos] = v[u];
-          c_i[w][pos] = id;
+          cd[pos] = v[u];
+          ci[pos] = vi[u];
         }
         cnt += __popcll(b);
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    if (cnt <= kTopkCap) {
-      // 3. four candidates per lane, sorted, then K pops.
-      float td[4];
-      uint32_t ti[4];
-      topk_clear<4>(td, ti);
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int q = lane + 64 * t;
-        if (q < cnt) topk_insert<4>(td, ti, c_d[w][q], c_i[w][q]);
-      }
-      wave_pop_list<4, K>(td, ti, od, oi, lane, cnt < kk ? cnt : kk);
-    } else {
-      float td[K];
-      uint32_t ti[K];
-      topk_clear<K>(td, ti);
-      // (rare: one column per lane at a time, so v[] is not live here)
-#pragma nounroll
-      for (int jl = lane; jl < nl; jl += 64) {
-        const float d = ps[jl];
-        const uint32_t id = idb + (uint32_t)jl;
-        if (!lex_less(thr_d, thr_i, d, id)) topk_insert<K>(td, ti, d, id);
-      }
-      wave_pop_list<K, K>(td, ti, od, oi, lane, kk);
-    }
-    // the next segment's stores to this LDS list must follow these reads
+    return cnt;
+  };
+  long cnt = E > 0 ? pack(false) : 0;
+  // 3. tighten while the packed entries do not all fit.
+  while (cnt > kTopkCap) {
+    topk_pop_packed<16>(cd, ci, kTopkCap, kk, lane, rd, ri, T_d, T_i);
+    // the next pack's stores to the list must follow these reads
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+    cnt = pack(true);
+  }
+  // 4. the result, in order.
+  float ld;
+  uint32_t li;
+  topk_pop_any(cd, ci, (int)cnt, kk, lane, rd, ri, ld, li);
+  np = (int)(cnt < kk ? cnt : kk);
+  // the next set's stores to this LDS list must follow these reads
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Stores a selection as a sorted list of lcap entries (the rest empty).
+__device__ __forceinline__ void store_list(float *od, uint32_t *oi, int lcap, int np, int lane,
+                                           const float (&rd)[2], const uint32_t (&ri)[2]) {
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int idx = 64 * s + lane;
+    if (idx < lcap) {
+      od[idx] = idx < np ? rd[s] : __builtin_inff();
+      oi[idx] = idx < np ? ri[s] : 0xFFFFFFFFu;
+    }
   }
 }
 
-// Stage 2: merges lists [l0, l1) of each row (`stride` sorted lists of K
-// per row), one wavefront per row, the same way stage 1 treats columns:
-// thr = the k_out-th smallest of the 64 lanes' smallest list heads (k_out
-// distinct entries, so the row's k_out best lie at or below it); the
-// entries at or below thr (a walk of each list that starts at or below thr)
-// are packed into an LDS list by an LDS counter, four per lane are sorted
-// and wavefront pops give the result: the row's best k_out into out_d /
-// out_i (when out_d is set, ids ~0 -> -1) or back into list 0 as a sorted
-// list (the rest of it empty).
-// More than kTopkCap such entries: every lane folds its lists' entries at or
-// below thr into a list of K instead (exact either way). Only lane 0 reads
-// list 0 and it does before any pop writes it.
-template <int K>
+// Stage 1 source: columns [0, nl) of one row segment through a buffer
+// resource (a 32-bit byte offset per lane, clamped to the segment's last
+// column in the vector offset itself, so no load leaves the segment whatever
+// part of the offset the hardware's range check covers); id = idb + column.
+struct SegSource {
+  __amdgpu_buffer_rsrc_t rs;
+  int nl;
+  uint32_t idb;
+  int lane;
+  __device__ __forceinline__ void pass(long p, float (&v)[kTopkPL], uint32_t (&vi)[kTopkPL]) const {
+#pragma unroll
+    for (int u = 0; u < kTopkPL; ++u) {
+      const int jl = (int)p * kTopkSub + u * 64 + lane;
+      const int jc = jl < nl ? jl : nl - 1;
+      v[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, 4 * jc, 0, 0));
+    }
+#pragma unroll
+    for (int u = 0; u < kTopkPL; ++u) {
+      const int jl = (int)p * kTopkSub + u * 64 + lane;
+      vi[u] = jl < nl ? idb + (uint32_t)jl : 0xFFFFFFFFu;
+      if (jl >= nl) v[u] = __builtin_inff();
+    }
+  }
+};
+
+// Stage 2 source: E entries of a row's consecutive lists (flat).
+struct ListSource {
+  const float *d;
+  const uint32_t *i;
+  long E;
+  int lane;
+  __device__ __forceinline__ void pass(long p, float (&v)[kTopkPL], uint32_t (&vi)[kTopkPL]) const {
+#pragma unroll
+    for (int u = 0; u < kTopkPL; ++u) {
+      const long e = p * kTopkSub + u * 64 + lane;
+      const long ec = e < E ? e : E - 1;
+      v[u] = d[ec];
+      vi[u] = i[ec];
+    }
+#pragma unroll
+    for (int u = 0; u < kTopkPL; ++u) {
+      const long e = p * kTopkSub + u * 64 + lane;
+      if (e >= E) {
+        v[u] = __builtin_inff();
+        vi[u] = 0xFFFFFFFFu;
+      }
+    }
+  }
+};
+
+// Stage 1: the m x nseg (row, segment) pairs of kTopkSeg columns, row-major,
+// go to the wavefronts in turn (wavefront w of workgroup x takes pairs
+// 4x + w, + 4 * gridDim.x, ...); a segment's best kk go to list seg0 + seg of
+// its row's `stride` lists of lcap entries (ids = column + id0). Only a row's
+// kk best are ever asked for, and they are among its segments' kk best. With
+// use_run, list 0 of the row holds the row's kk best over the columns before
+// this block (ids below every id here), and its kk-th entry bounds T.
+__global__ __launch_bounds__(256) void k_topk_rows(const float *__restrict__ dist, long m, long n,
+                                                   long nseg, long stride, long seg0, uint32_t id0,
+                                                   int kk, int lcap, int use_run,
+                                                   float *__restrict__ seg_d,
+                                                   uint32_t *__restrict__ seg_i) {
+  __shared__ float c_d[4][kTopkCap];
+  __shared__ uint32_t c_i[4][kTopkCap];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const long pairs = m * nseg;
+  for (long t = (long)blockIdx.x * 4 + w; t < pairs; t += (long)gridDim.x * 4) {
+    const long row = t / nseg, seg = t - row * nseg;
+    float *od = seg_d + (row * stride + seg0 + seg) * lcap;
+    uint32_t *oi = seg_i + (row * stride + seg0 + seg) * lcap;
+    float rd[2];
+    uint32_t ri[2];
+    int np = 0;
+    if (n > 0) {  // no columns (dist may be null): an empty list
+      const long c0 = seg * kTopkSeg;
+      const int nl = (int)(n - c0 < kTopkSeg ? n - c0 : kTopkSeg);  // >= 1
+      SegSource src{__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(dist + row * n + c0), 0,
+                                                      4 * nl, kBufferWord3),
+                    nl, (uint32_t)c0 + id0, lane};
+      float hd = __builtin_inff();
+      uint32_t hi = 0xFFFFFFFFu;
+      if (use_run) {
+        hd = seg_d[row * stride * lcap + kk - 1];
+        hi = seg_i[row * stride * lcap + kk - 1];
+      }
+      wave_select(src, nl, kk, hd, hi, c_d[w], c_i[w], lane, rd, ri, np);
+    }
+    store_list(od, oi, lcap, np, lane, rd, ri);
+  }
+}
+
+// Stage 2: merges lists [l0, l1) of each row (`stride` sorted lists of lcap
+// per row), one wavefront per row, into the row's best k_out: into out_d /
+// out_i (when out_d is set; ids ~0 -> -1) or back into list 0 as a sorted
+// list. A row whose lists past list 0 are all empty, when list 0 is one of
+// the inputs and the output, keeps list 0 as it is. Every read of the lists
+// precedes the first write.
 __global__ __launch_bounds__(256) void k_topk_rows_fold(long m, long stride, long l0, long l1,
-                                                        int k_out, float *__restrict__ seg_d,
+                                                        int k_out, int lcap,
+                                                        float *__restrict__ seg_d,
                                                         uint32_t *__restrict__ seg_i,
                                                         float *__restrict__ out_d,
                                                         int64_t *__restrict__ out_i) {
   __shared__ float c_d[4][kTopkCap];
   __shared__ uint32_t c_i[4][kTopkCap];
-  __shared__ int c_n[4];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const long row = (long)blockIdx.x * 4 + w;
   if (row >= m) return;  // no barrier below
-  const float *rd = seg_d + row * stride * K;
-  const uint32_t *ri = seg_i + row * stride * K;
-  const long nl = l1 - l0;
-  // 1. the lane's smallest head; thr. A row whose lists past list 0 are all
-  // empty (later chunks of a blocked search, mostly) keeps list 0 as it is.
-  float md = __builtin_inff();
-  uint32_t mi = 0xFFFFFFFFu;
-  bool fresh = false;  // a real entry in a list other than list 0
-  for (long b = 0; b < nl; b += 64 * 8) {
-    float hd[8];
-    uint32_t hi[8];
+  float *rd0 = seg_d + row * stride * lcap;
+  uint32_t *ri0 = seg_i + row * stride * lcap;
+  if (!out_d && l0 == 0) {
+    bool fresh = false;  // a real entry in a list other than list 0
+    for (long l = 1 + lane; l < l1; l += 64) fresh = fresh || ri0[l * lcap] != 0xFFFFFFFFu;
+    if (!__any(fresh)) return;  // wavefront-uniform
+  }
+  const ListSource src{rd0 + l0 * lcap, ri0 + l0 * lcap, (l1 - l0) * lcap, lane};
+  float rd[2];
+  uint32_t ri[2];
+  int np = 0;
+  wave_select(src, src.E, k_out, __builtin_inff(), 0xFFFFFFFFu, c_d[w], c_i[w], lane, rd, ri, np);
+  if (out_d) {
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const long l = b + t * 64 + lane;
-      const long lc = l0 + (l < nl ? l : nl - 1);
-      hd[t] = rd[lc * K];
-      hi[t] = ri[lc * K];
-    }
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const long l = b + t * 64 + lane;
-      if (l < nl && l0 + l != 0 && hi[t] != 0xFFFFFFFFu) fresh = true;
-      if (l < nl && lex_less(hd[t], hi[t], md, mi)) {
-        md = hd[t];
-        mi = hi[t];
+    for (int s = 0; s < 2; ++s) {
+      const int idx = 64 * s + lane;
+      if (idx < k_out) {
+        out_d[row * k_out + idx] = idx < np ? rd[s] : __builtin_inff();
+        out_i[row * k_out + idx] = idx < np && ri[s] != 0xFFFFFFFFu ? (int64_t)ri[s] : -1;
       }
-    }
-  }
-  // Nothing new to fold into list 0: keep it as it is. Only when list 0 is
-  // one of the inputs (l0 == 0); otherwise list 0 is this fold's output and
-  // is always written. Wavefront-uniform; no barrier below.
-  if (!out_d && l0 == 0 && !__any(fresh)) return;
-  float thr_d = __builtin_inff();
-  uint32_t thr_i = 0xFFFFFFFFu;
-  for (int r = 0; r < k_out; ++r) {
-    float bd = md;
-    uint32_t bi = mi;
-    wave_lex_min(bd, bi);
-    thr_d = bd;
-    thr_i = bi;
-    if (md == bd && mi == bi) {
-      md = __builtin_inff();
-      mi = 0xFFFFFFFFu;
-    }
-  }
-  // 2. entries at or below thr.
-  if (lane == 0) c_n[w] = 0;
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  for (long l = lane; l < nl; l += 64) {
-    const float *ld = rd + (l0 + l) * K;
-    const uint32_t *li = ri + (l0 + l) * K;
-    for (int k = 0; k < K; ++k) {
-      const float d = ld[k];
-      const uint32_t id = li[k];
-      if (lex_less(thr_d, thr_i, d, id) || id == 0xFFFFFFFFu) break;  // sorted: the rest is above
-      const int pos = atomicAdd(&c_n[w], 1);
-      if (pos < kTopkCap) {
-        c_d[w][pos] = d;
-        c_i[w][pos] = id;
-      }
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  const int cnt = c_n[w];
-  if (cnt <= kTopkCap) {
-    float td[4];
-    uint32_t ti[4];
-    topk_clear<4>(td, ti);
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int q = lane + 64 * t;
-      if (q < cnt) topk_insert<4>(td, ti, c_d[w][q], c_i[w][q]);
-    }
-    if (out_d) {
-      wave_pop_topk<4>(td, ti, k_out, lane, out_d + row * k_out, out_i + row * k_out);
-    } else {
-      wave_pop_list<4, K>(td, ti, seg_d + row * stride * K, seg_i + row * stride * K, lane,
-                          cnt < k_out ? cnt : k_out);
     }
   } else {
-    float td[K];
-    uint32_t ti[K];
-    topk_clear<K>(td, ti);
-    for (long l = lane; l < nl; l += 64)
-      take_sorted<K>(td, ti, rd + (l0 + l) * K, ri + (l0 + l) * K);
-    if (out_d) {
-      wave_pop_topk<K>(td, ti, k_out, lane, out_d + row * k_out, out_i + row * k_out);
-    } else {
-      wave_pop_list<K, K>(td, ti, seg_d + row * stride * K, seg_i + row * stride * K, lane, k_out);
-    }
+    store_list(rd0, ri0, lcap, np, lane, rd, ri);
   }
 }
 
@@ -837,14 +852,14 @@ long topk_rows_segments(long m, long n) {
 hipError_t launch_topk_rows(const float *dist, long m, long n, int k, float *out_d,
                             int64_t *out_i, float *seg_d, uint32_t *seg_i, hipStream_t st) {
   if (m <= 0) return hipSuccess;
-  if (k < 1 || k > kSearchMaxK || m > 65535 || n >= 0xFFFFFFFFL) return hipErrorInvalidValue;
+  if (k < 1 || k > kTopkMaxK || m > 65535 || n >= 0xFFFFFFFFL) return hipErrorInvalidValue;
   const long nseg = topk_rows_segments(m, n);
-  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3(topk_rows_grid(m, nseg)), dim3(256), 0, st,
-                     dist, m, n, nseg, nseg, 0L, 0u, k, 0, seg_d, seg_i);
+  hipLaunchKernelGGL(k_topk_rows, dim3(topk_rows_grid(m, nseg)), dim3(256), 0, st, dist, m, n,
+                     nseg, nseg, 0L, 0u, k, k, 0, seg_d, seg_i);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_topk_rows_fold<kSearchMaxK>, dim3((unsigned)((m + 3) / 4)), dim3(256), 0,
-                     st, m, nseg, 0L, nseg, k, seg_d, seg_i, out_d, out_i);
+  hipLaunchKernelGGL(k_topk_rows_fold, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, st, m, nseg,
+                     0L, nseg, k, k, seg_d, seg_i, out_d, out_i);
   return hipGetLastError();
 }
 
@@ -853,20 +868,20 @@ hipError_t launch_topk_rows_chunk(const float *dist, long m, long n, long stride
                                   uint32_t *seg_i, hipStream_t st) {
   if (m <= 0 || n <= 0) return hipSuccess;
   const long nseg = topk_rows_segments(m, n);
-  if (k < 1 || k > kSearchMaxK || m > 65535 || n >= 0xFFFFFFFFL || seg0 + nseg > stride ||
+  if (k < 1 || k > kTopkMaxK || m > 65535 || n >= 0xFFFFFFFFL || seg0 + nseg > stride ||
       (use_run && seg0 < 1))
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_topk_rows<kSearchMaxK>, dim3(topk_rows_grid(m, nseg)), dim3(256), 0, st,
-                     dist, m, n, nseg, stride, seg0, id0, k, use_run ? 1 : 0, seg_d, seg_i);
+  hipLaunchKernelGGL(k_topk_rows, dim3(topk_rows_grid(m, nseg)), dim3(256), 0, st, dist, m, n,
+                     nseg, stride, seg0, id0, k, k, use_run ? 1 : 0, seg_d, seg_i);
   return hipGetLastError();
 }
 
 hipError_t launch_topk_rows_fold(long m, long stride, long l0, long l1, int k, float *seg_d,
                                  uint32_t *seg_i, float *out_d, int64_t *out_i, hipStream_t st) {
   if (m <= 0) return hipSuccess;
-  if (k < 1 || k > kSearchMaxK || l0 < 0 || l1 > stride || l0 >= l1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_topk_rows_fold<kSearchMaxK>, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, st,
-                     m, stride, l0, l1, k, seg_d, seg_i, out_d, out_i);
+  if (k < 1 || k > kTopkMaxK || l0 < 0 || l1 > stride || l0 >= l1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_topk_rows_fold, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, st, m, stride,
+                     l0, l1, k, k, seg_d, seg_i, out_d, out_i);
   return hipGetLastError();
 }
 

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "tq_internal.h"
+#include "tq_search.h"
 
 namespace {
 
@@ -227,13 +228,15 @@ size_t batch_bound(tq_server_s *s, const std::vector<Request *> &b) {
     bytes += align_up(r->rows * in.dim * sizeof(float)) + 2048;
     if (r->kind != kSearch) continue;
     any = true;
-    bytes += r->rows * (in.dim * 4 + in.prepared_stride * 4 + in.padded_dim * 6 +
-                        (size_t)s->cfg.k * 24) + 8 * 256;
+    // split query planes: at most 4 fp16 planes over rows rounded up to 32
+    bytes += r->rows * (in.dim * 4 + in.prepared_stride * 4 + in.padded_dim * 8 +
+                        (size_t)s->cfg.k * 24) + 31 * in.padded_dim * 8 + 8 * 256;
     qtiles += (r->rows + 31) / 32;
   }
   if (any) {
     if (s->cfg.mode == TQ_MODE_MFMA) {
-      bytes += (3 * (size_t)s->cus * 8 + qtiles + 16) * 4 * 64 * TQ_SEARCH_MAX_K * 8;
+      if (s->cfg.k <= tq::kSearchMaxK)  // fused-form candidate lists
+        bytes += (3 * (size_t)s->cus * 8 + qtiles + 16) * 4 * 64 * tq::kSearchMaxK * 8;
       long rows = 0, n_max = 0;
       bool blocked = false;
       for (Request *r : b)
@@ -242,9 +245,11 @@ size_t batch_bound(tq_server_s *s, const std::vector<Request *> &b) {
           n_max = std::max(n_max, (long)r->n_at);
           blocked = blocked || tq::plan_blocked((long)r->n_at, 1);
         }
-      if (blocked) bytes += tq::plan_blocked_ws_bound(rows, n_max);
+      if (blocked) bytes += tq::plan_blocked_ws_bound(rows, n_max, s->cfg.k);
     } else
-      bytes += (512ull << 20) + (8ull << 20);
+      // a 512 MiB distance block and its row top-k lists: at most
+      // (2^27 / 8192 + 65535) lists of k entries of 8 bytes
+      bytes += (512ull << 20) + (size_t)s->cfg.k * (1ull << 20);
   }
   return bytes;
 }
@@ -410,6 +415,9 @@ int tq_server_create(const tq_server_config *cfg, tq_server *out) {
   if (!c.max_inflight) c.max_inflight = 2;
   if (!c.max_batch_queries) c.max_batch_queries = 4096;
   if (c.max_inflight > 8) return fail(TQ_EINVAL, "%s: max_inflight must be at most 8", what);
+  if (c.mode == TQ_MODE_MFMA && c.k > tq::kSearchMaxK && c.max_batch_queries > 65535)
+    return fail(TQ_EINVAL, "%s: k > %d runs blocked, which takes at most 65535 queries per batch",
+                what, tq::kSearchMaxK);
   const uint64_t reserved = (uint64_t)c.max_inflight * c.slot_bytes;
   if (reserved > c.hbm_budget)
     return fail(TQ_ENOMEM, "%s: %u slots of %llu bytes exceed the budget of %llu bytes", what,

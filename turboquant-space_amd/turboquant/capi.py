@@ -39,7 +39,7 @@ EXPORTED = [
     "tq_server_submit_search", "tq_server_wait", "tq_server_collection", "tq_server_load_codes", "tq_server_read_codes",
     "tq_server_stats_get",
 ]
-TQ_SEARCH_MAX_K = 16
+TQ_SEARCH_MAX_K = 128
 TQ_FORM_AUTO, TQ_FORM_FUSED, TQ_FORM_BLOCKED = 0, 1, 2
 
 
