@@ -105,6 +105,23 @@ int tq_distance_m_to_n(tq_space s, const float *Q, uint64_t m,
                        const uint8_t *codes, uint64_t n, float *out, int mode,
                        int mem, void *stream);
 
+/* No counterpart in the reference: tq_distance_m_to_n for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Device buffers only; the call
+ * enqueues the same kernels on `stream` and returns without waiting. Its
+ * scratch (prepared queries, then the MFMA query planes or the fp32 scan's
+ * QJL digit words) is one fixed area of TQ_NOALLOC_SCRATCH_BYTES per device
+ * inside the library, held until the process ends: calls needing more
+ * (about 16 KB per query at padded dim 1024, i.e. up to ~4,000 queries) fail
+ * with TQ_ENOMEM. Every call on a device shares that area, so calls that
+ * overlap on one device (different streams, other threads) race: order them,
+ * e.g. on one stream. No allocation, host synchronisation or library
+ * allocation of any kind happens in the call. */
+#define TQ_NOALLOC_SCRATCH_BYTES (64u << 20)
+int tq_distance_m_to_n_noalloc(tq_space s, const float *Q, uint64_t m,
+                               const uint8_t *codes, uint64_t n, float *out,
+                               int mode, void *stream);
+
 /* The scan of distanceBatchMToN without the per-query preparation (the loop
  * body of space_turbo_quant.h:592-598): prepared queries from
  * tq_prepare_queries, device memory only. */

@@ -16,7 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = [
     "tq_capi.hip", "tq_encode.hip", "tq_scan.hip", "tq_lut.hip", "tq_mfma.hip", "tq_search.hip",
-    "tq_plan.hip", "tq_serve.hip", "tq_symfull.hip",
+    "tq_plan.hip", "tq_serve.hip", "tq_symfull.hip", "tq_noalloc.hip",
 ]
 # -ffp-contract=off: the encoder reproduces the reference's fp32 rounding
 # exactly, so every fused multiply-add in the kernels is written explicitly.

@@ -29,7 +29,8 @@ TQ_SYM_LIGHT = 2
 EXPORTED = [
     "tq_last_error", "tq_device_count", "tq_space_create", "tq_space_destroy",
     "tq_space_info_get", "tq_space_tables", "tq_encode_batch", "tq_prepare_queries",
-    "tq_distance_1_to_n", "tq_distance_m_to_n", "tq_distance_prepared",
+    "tq_distance_1_to_n", "tq_distance_m_to_n", "tq_distance_m_to_n_noalloc",
+    "tq_distance_prepared",
     "tq_distance_m_to_n_symmetric", "tq_distance_m_to_n_symmetric_ex", "tq_prepare_symmetric",
     "tq_distance_symmetric_prepared", "tq_rotated_coords_batch", "tq_encode_graph_create",
     "tq_encode_graph_run", "tq_encode_graph_destroy", "tq_search_plan_create", "tq_search_plan_run",
@@ -128,6 +129,8 @@ def lib():
         L.tq_distance_1_to_n.argtypes = [P, P, P, U64, P, I, P]
         L.tq_distance_m_to_n.restype = I
         L.tq_distance_m_to_n.argtypes = [P, P, U64, P, U64, P, I, I, P]
+        L.tq_distance_m_to_n_noalloc.restype = I
+        L.tq_distance_m_to_n_noalloc.argtypes = [P, P, U64, P, U64, P, I, P]
         L.tq_distance_prepared.restype = I
         L.tq_distance_prepared.argtypes = [P, P, U64, P, U64, P, I, P]
         L.tq_distance_m_to_n_symmetric.restype = I
@@ -246,6 +249,10 @@ class Space:
     def m_to_n(self, Q, m, codes, n, out, mode=TQ_MODE_FP32, mem=TQ_MEM_DEVICE, stream=None):
         check(lib().tq_distance_m_to_n(self.h, ptr(Q), m, ptr(codes), n, ptr(out), mode, mem,
                                        stream))
+
+    def m_to_n_noalloc(self, Q, m, codes, n, out, mode=TQ_MODE_FP32, stream=None):
+        check(lib().tq_distance_m_to_n_noalloc(self.h, ptr(Q), m, ptr(codes), n, ptr(out), mode,
+                                               stream))
 
     def one_to_n(self, q, codes, n, out, mem=TQ_MEM_DEVICE, stream=None):
         check(lib().tq_distance_1_to_n(self.h, ptr(q), ptr(codes), n, ptr(out), mem, stream))
