@@ -236,11 +236,31 @@ def test_mfma_against_fp64_estimator(gpu, torch_cuda, dim, bits, m, n):
     assert scale_rel_err(fp32, want, qns, xns).max() <= FP32_TOL
 
 
-def test_mfma_rejects_tiny_dims(gpu, torch_cuda):
-    # the MFMA form walks the padded dim in 32-coordinate chunks
+@pytest.mark.parametrize("dim,bits", [(16, 4), (3, 4), (9, 8), (12, 3)])
+def test_mfma_mode_below_the_tile(gpu, torch_cuda, dim, bits):
+    # Padded dims below 32 have no 32-coordinate MFMA chunk: MFMA mode serves
+    # them with the plain fp32 kernels (asymmetric and symmetric), so its
+    # distances equal the fp32 mode's bit for bit and meet its bar.
     capi, torch = gpu, torch_cuda
-    sp = capi.Space(16, 4)
-    Q = np.zeros((1, 16), np.float32)
-    codes = np.zeros((1, sp.cs), np.uint8)
-    with pytest.raises(capi.TQError):
-        gpu_m_to_n(torch, capi, sp, Q, codes, capi.TQ_MODE_MFMA)
+    say(f"[gpu] MFMA mode below the tile dim={dim} bits={bits}")
+    sp = capi.Space(dim, bits)
+    o = Oracle(dim, bits)
+    rng = np.random.default_rng(dim * 31 + bits)
+    X = rng.standard_normal((3000, dim)).astype(np.float32)
+    Q = rng.standard_normal((40, dim)).astype(np.float32)
+    codes = gpu_encode(torch, capi, sp, X)
+    got = gpu_m_to_n(torch, capi, sp, Q, codes, capi.TQ_MODE_MFMA)
+    fp32 = gpu_m_to_n(torch, capi, sp, Q, codes, capi.TQ_MODE_FP32)
+    np.testing.assert_array_equal(got.view(np.uint32), fp32.view(np.uint32))
+    qns = np.sum(Q.astype(np.float64) ** 2, 1)
+    xns = o.trailer(codes)[:, 0].astype(np.float64) ** 2
+    exact = o.m_to_n_f64(Q, codes)
+    assert fp32_parity_excess(got, o.m_to_n(Q, codes), exact, qns, xns).max() <= FP32_TOL
+    dA, dB = dev(torch, codes[:50]), dev(torch, codes)
+    out_m = torch.empty((50, len(codes)), dtype=torch.float32, device="cuda")
+    out_f = torch.empty_like(out_m)
+    sp.symmetric(dA, 50, dB, len(codes), out_m, mode=capi.TQ_MODE_MFMA)
+    sp.symmetric(dA, 50, dB, len(codes), out_f)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out_m.cpu().numpy().view(np.uint32),
+                                  out_f.cpu().numpy().view(np.uint32))

@@ -326,7 +326,6 @@ def test_topk_rows_ties_by_column(gpu, torch_cuda, kind):
 def test_search_rejects_bad_arguments(gpu, torch_cuda):
     torch, capi = torch_cuda, gpu
     c = Coll(torch, capi, 128, 4, 10, 51)
-    small = capi.Space(16, 4)  # padded dim 16: below the MFMA tile
     _, dQ = c.queries(torch, 2)
     od, oi = outputs(torch, 2, 16)
     for k in (0, 129):
@@ -337,7 +336,33 @@ def test_search_rejects_bad_arguments(gpu, torch_cuda):
         capi.SearchPlan([(c.sp, dQ, 2, c.codes, 10, od17, oi17)], 17, form=capi.TQ_FORM_FUSED)
     with pytest.raises(capi.TQError, match="query_tile"):
         capi.SearchPlan([(c.sp, dQ, 2, c.codes, 10, od, oi)], 10, query_tile=48)
-    with pytest.raises(capi.TQError, match="padded dim"):
-        capi.SearchPlan([(small, dQ, 2, c.codes, 10, od, oi)], 10)
     with pytest.raises(capi.TQError, match="NULL"):
         capi.SearchPlan([(c.sp, None, 2, c.codes, 10, od, oi)], 10)
+
+
+@pytest.mark.parametrize("dim", [16, 5])
+def test_mfma_plan_below_the_tile(gpu, torch_cuda, dim):
+    # A collection of padded dim below 32 in an MFMA plan runs the plain fp32
+    # scan and row top-k, next to an MFMA batch in the same plan: its result
+    # equals the fp32 plan's, and the MFMA job's equals its own alone.
+    torch, capi = torch_cuda, gpu
+    say(f"[gpu] MFMA plan with a dim-{dim} collection")
+    small = Coll(torch, capi, dim, 4, 5000, 61)
+    big = Coll(torch, capi, 128, 4, 7000, 62)
+    _, qs = small.queries(torch, 9)
+    _, qb = big.queries(torch, 33)
+    outs = {}
+    for mode in (capi.TQ_MODE_MFMA, capi.TQ_MODE_FP32):
+        od, oi = outputs(torch, 9, 10)
+        bd, bi = outputs(torch, 33, 10)
+        capi.check(capi.lib().tq_search(capi.make_jobs(
+            [(small.sp, qs, 9, small.codes, small.n, od, oi),
+             (big.sp, qb, 33, big.codes, big.n, bd, bi)]), 2, 10, mode, None))
+        torch.cuda.synchronize()
+        outs[mode] = (od.cpu().numpy(), oi.cpu().numpy(), bd.cpu().numpy(), bi.cpu().numpy())
+    m, f = outs[capi.TQ_MODE_MFMA], outs[capi.TQ_MODE_FP32]
+    np.testing.assert_array_equal(m[1], f[1])
+    np.testing.assert_array_equal(m[0].view(np.uint32), f[0].view(np.uint32))
+    want_d, want_i = host_topk(full_matrix(torch, capi, big, qb, 33, capi.TQ_MODE_MFMA), 10)
+    np.testing.assert_array_equal(m[3], want_i)
+    np.testing.assert_array_equal(m[2], want_d)

@@ -4,8 +4,8 @@
 
 namespace tq {
 // out[q*n + j] for q < m, j < n from prepared queries (prep_stride floats
-// each) and codes; returns hipErrorNotSupported for padded dims it does not
-// handle. qjl = false: the queries' s_q are all zero (symmetric MSE
+// each) and codes. Padded dims below 32 (no MFMA tile) run the plain fp32
+// kernels. qjl = false: the queries' s_q are all zero (symmetric MSE
 // estimator) and the QJL passes are skipped.
 hipError_t launch_scan_mfma(const SpaceDev &s, const float *prep, long m,
                             const uint8_t *codes, long n, float *out,

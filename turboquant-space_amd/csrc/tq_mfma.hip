@@ -396,8 +396,10 @@ hipError_t launch_mfma_planes(const SpaceDev &s, const _Float16 *planes, const f
 
 hipError_t launch_scan_mfma(const SpaceDev &s, const float *prep, long m, const uint8_t *codes,
                             long n, float *out, hipStream_t st, bool qjl) {
-  if (s.D < kKC) return hipErrorNotSupported;
   if (m <= 0 || n <= 0) return hipSuccess;
+  // Below one 32-coordinate chunk there is no MFMA tile: the plain fp32
+  // kernels serve the call (same estimator, fp32 accuracy).
+  if (s.D < kKC) return launch_scan_fp32(s, prep, m, codes, n, out, !qjl, st);
   // grid.y covers 64 queries per workgroup row; larger batches go in chunks.
   const long chunk_max = 65535L * 64;
   const long chunk = m < chunk_max ? m : chunk_max;

@@ -114,6 +114,10 @@ struct PlanBatch {
   std::vector<long> job_row;  // first row of each job within the batch
   int qr = 1;                 // query tile: 32 * qr
   bool blocked = false;       // MFMA plans: k_mfma blocks + row top-k instead of k_search
+  // The plain fp32 scan in row blocks + row top-k: every batch of an FP32
+  // plan, and batches of padded dim below 32 (under the MFMA tile's
+  // 32-coordinate chunk) in MFMA plans.
+  bool plain = false;
   long nc = 0;                // blocked: codes per chunk
   long lists = 0;             // blocked: candidate lists per query (1 + segments of a chunk)
   // device pointers inside the workspace
@@ -242,6 +246,7 @@ int layout(tq_search_plan_s *p) {
   for (PlanBatch &b : p->batches) {
     const tq_search_tuning t = knobs_for(p, b);
     b.qr = t.query_tile == 64 ? 2 : 1;
+    b.plain = p->mode == TQ_MODE_FP32 || b.space->info.padded_dim < 32;
     // k above the fused form's list length always runs blocked (plan_new
     // refuses plans where that is not possible).
     b.blocked = p->mode == TQ_MODE_MFMA &&
@@ -269,7 +274,7 @@ int layout(tq_search_plan_s *p) {
     for (int kind = 1; kind <= 2; ++kind) {
       double total = 0.0;
       for (const PlanBatch &b : p->batches) {
-        if (b.qr != kind || b.m == 0 || b.blocked) continue;
+        if (b.qr != kind || b.m == 0 || b.blocked || b.plain) continue;
         const long qtiles = (b.m + 32 * kind - 1) / (32 * kind);
         const long ntiles = (b.n + ntile - 1) / ntile;
         total += (double)qtiles * ntiles * b.space->info.padded_dim * kind;
@@ -278,7 +283,7 @@ int layout(tq_search_plan_s *p) {
       const double target = total / (double)(wg * ipw);
       for (size_t bi = 0; bi < p->batches.size(); ++bi) {
         const PlanBatch &b = p->batches[bi];
-        if (b.qr != kind || b.m == 0 || b.blocked) continue;
+        if (b.qr != kind || b.m == 0 || b.blocked || b.plain) continue;
         const long qtiles = (b.m + 32 * kind - 1) / (32 * kind);
         const long ntiles = (b.n + ntile - 1) / ntile;
         const double tile_cost = (double)b.space->info.padded_dim * kind;
@@ -344,7 +349,7 @@ int layout(tq_search_plan_s *p) {
     const tq_space_info &info = b.space->info;
     p->o_q[bi] = take((size_t)b.m * info.dim * sizeof(float));
     p->o_prep[bi] = take((size_t)b.m * info.prepared_stride * sizeof(float));
-    if (p->mode == TQ_MODE_MFMA) {
+    if (!b.plain) {
       p->o_planes[bi] = take(tq::split_planes_bytes(b.space->dev, (long)b.m));
       if (b.blocked) {
         long nc = (long)(tq::kMfmaBlockBytes / (sizeof(float) * (size_t)b.m));
@@ -405,7 +410,7 @@ int bind(tq_search_plan_s *p, void *ws, hipStream_t st, bool async) {
     b.queries = reinterpret_cast<float *>(w + p->o_q[bi]);
     b.prep = reinterpret_cast<float *>(w + p->o_prep[bi]);
     b.planes =
-        p->mode == TQ_MODE_MFMA ? reinterpret_cast<_Float16 *>(w + p->o_planes[bi]) : nullptr;
+        !b.plain ? reinterpret_cast<_Float16 *>(w + p->o_planes[bi]) : nullptr;
     tq::SearchJobDev jd{};
     jd.s = b.space->dev;
     jd.planes = b.planes;
@@ -487,7 +492,7 @@ int enqueue(tq_search_plan_s *p, hipStream_t st) {
     if (b.m == 0) continue;
     if ((e = tq::launch_prepare(b.space->dev, b.queries, b.m, b.prep, st)) != hipSuccess)
       return hip_fail(e, what);
-    if (p->mode == TQ_MODE_MFMA &&
+    if (!b.plain &&
         (e = tq::launch_split_queries(b.space->dev, b.prep, b.m, b.planes, st)) != hipSuccess)
       return hip_fail(e, what);
   }
@@ -520,9 +525,10 @@ int enqueue(tq_search_plan_s *p, hipStream_t st) {
           return hip_fail(e, what);
       }
     }
-  } else {
+  }
+  {
     for (const PlanBatch &b : p->batches) {
-      if (b.m == 0) continue;
+      if (b.m == 0 || !b.plain) continue;
       float *od = p->all_d + b.row0 * p->k;
       int64_t *oi = p->all_i + b.row0 * p->k;
       // With no codes the scan is skipped and the top-k writes empty entries.
@@ -588,9 +594,6 @@ int plan_new(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
       return fail(TQ_EINVAL, "%s: job %llu: m must be at most 2^31", what, (unsigned long long)i);
     if (j.m > 0 && (!j.queries || !j.out_dist || !j.out_ids || (j.n > 0 && !j.codes)))
       return fail(TQ_EINVAL, "%s: job %llu has a NULL buffer", what, (unsigned long long)i);
-    if (mode == TQ_MODE_MFMA && j.space->dev.D < 32)
-      return fail(TQ_EUNSUPPORTED, "%s: job %llu: MFMA search needs padded dim >= 32, got %d", what,
-                  (unsigned long long)i, j.space->dev.D);
   }
   // k above the fused form's per-lane lists runs blocked only.
   if (mode == TQ_MODE_MFMA && k > tq::kSearchMaxK) {
@@ -744,7 +747,7 @@ int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen
     // Each collection batch alone: a sub-plan over that batch's jobs.
     for (size_t bi = 0; bi < p->batches.size() && rc == TQ_OK; ++bi) {
       const PlanBatch &b = p->batches[bi];
-      if (b.m == 0 || b.n == 0) continue;
+      if (b.m == 0 || b.n == 0 || b.plain) continue;  // plain batches have no knobs
       std::vector<tq_search_job> sub;
       for (int ji : b.jobs) sub.push_back(p->jobs[ji]);
       // Candidates: the fused form's knobs, then the blocked form (when the

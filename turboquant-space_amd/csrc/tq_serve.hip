@@ -233,6 +233,16 @@ size_t batch_bound(tq_server_s *s, const std::vector<Request *> &b) {
                         (size_t)s->cfg.k * 24) + 31 * in.padded_dim * 8 + 8 * 256;
     qtiles += (r->rows + 31) / 32;
   }
+  // Searches the plan runs plain (fp32 scan in row blocks + row top-k): all of
+  // an FP32 server's, and those over collections of padded dim below 32.
+  bool plain = false;
+  for (Request *r : b)
+    plain = plain || (r->kind == kSearch && (s->cfg.mode == TQ_MODE_FP32 ||
+                                             r->space->info.padded_dim < 32));
+  if (plain)
+    // a 512 MiB distance block and its row top-k lists: at most
+    // (2^27 / 8192 + 65535) lists of k entries of 8 bytes
+    bytes += (512ull << 20) + (size_t)s->cfg.k * (1ull << 20);
   if (any) {
     if (s->cfg.mode == TQ_MODE_MFMA) {
       if (s->cfg.k <= tq::kSearchMaxK)  // fused-form candidate lists
@@ -246,10 +256,7 @@ size_t batch_bound(tq_server_s *s, const std::vector<Request *> &b) {
           blocked = blocked || tq::plan_blocked((long)r->n_at, 1);
         }
       if (blocked) bytes += tq::plan_blocked_ws_bound(rows, n_max, s->cfg.k);
-    } else
-      // a 512 MiB distance block and its row top-k lists: at most
-      // (2^27 / 8192 + 65535) lists of k entries of 8 bytes
-      bytes += (512ull << 20) + (size_t)s->cfg.k * (1ull << 20);
+    }
   }
   return bytes;
 }
@@ -475,8 +482,6 @@ int tq_server_add_collection(tq_server s, tq_space space, uint64_t capacity, uin
   if (space->device != s->device)
     return fail(TQ_EINVAL, "%s: space is on device %d, the server on %d", what, space->device,
                 s->device);
-  if (s->cfg.mode == TQ_MODE_MFMA && space->dev.D < 32)
-    return fail(TQ_EUNSUPPORTED, "%s: MFMA search needs padded dim >= 32", what);
   const uint64_t bytes = capacity * space->info.code_size;
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->reserved + s->resident + bytes > s->cfg.hbm_budget)
