@@ -564,3 +564,58 @@ void tqo_distance_m_to_n_symmetric_f64(const tqo_space *s, const uint8_t *A, siz
     }
   }
 }
+
+/* FP64 restatements of the symmetric QJL-corrected estimators (full,
+ * space_turbo_quant.h:731-800, whose prepared form :818-891 is the same
+ * quantity; light, :876-924), over m x n code pairs. */
+void tqo_distance_m_to_n_symmetric_full_f64(const tqo_space *s, const uint8_t *A, size_t m,
+                                            const uint8_t *B, size_t n, int light,
+                                            double *out) {
+  const size_t D = s->dim;
+  const double scale = sqrt(M_PI / 2.0) / sqrt((double)D);
+  const double pi_over_2d = M_PI / (2.0 * (double)D);
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)m; ++i) {
+    const uint8_t *a = A + (size_t)i * s->code_size;
+    float ma[3];
+    code_meta(s, a, ma);
+    double *ra = (double *)malloc(D * sizeof(double)), *rb = (double *)malloc(D * sizeof(double));
+    double *ha = (double *)malloc(D * sizeof(double)), *hb = (double *)malloc(D * sizeof(double));
+    for (size_t k = 0; k < D; ++k) {
+      ra[k] = (double)s->centroids[code_unit(s, a, k) >> 1] * (double)ma[2];
+      ha[k] = ra[k] * s->qjl_signs[k];
+    }
+    wht64(ha, D);
+    for (size_t j = 0; j < n; ++j) {
+      const uint8_t *b = B + j * s->code_size;
+      float mb[3];
+      code_meta(s, b, mb);
+      for (size_t k = 0; k < D; ++k) {
+        rb[k] = (double)s->centroids[code_unit(s, b, k) >> 1] * (double)mb[2];
+        hb[k] = rb[k] * s->qjl_signs[k];
+      }
+      wht64(hb, D);
+      double ip_mse = 0.0, d_ab = 0.0, d_ba = 0.0, d_ss = 0.0;
+      for (size_t k = 0; k < D; ++k) {
+        const double sa = (code_unit(s, a, k) & 1) ? 1.0 : -1.0;
+        const double sb = (code_unit(s, b, k) & 1) ? 1.0 : -1.0;
+        ip_mse += ra[k] * rb[k];
+        d_ab += ha[k] * sb;
+        d_ba += sa * hb[k];
+        d_ss += sa * sb;
+      }
+      const double t4 = pi_over_2d * (double)ma[1] * (double)mb[1] * d_ss;
+      double corr = light ? t4
+                          : scale * (double)mb[1] * d_ab + scale * (double)ma[1] * d_ba + t4;
+      const double mc = fabs(ip_mse);
+      corr = corr < -mc ? -mc : (corr > mc ? mc : corr);
+      const double ip = (ip_mse + corr) * (double)ma[0] * (double)mb[0];
+      const double d = (double)ma[0] * ma[0] + (double)mb[0] * mb[0] - 2.0 * ip;
+      out[(size_t)i * n + j] = d > 0.0 ? d : 0.0;
+    }
+    free(ra);
+    free(rb);
+    free(ha);
+    free(hb);
+  }
+}

@@ -81,6 +81,10 @@ void tqo_distance_m_to_n_f64(const tqo_space *s, const float *Q, size_t m,
                              const uint8_t *codes, size_t n, double *out);
 void tqo_distance_m_to_n_symmetric_f64(const tqo_space *s, const uint8_t *A, size_t m,
                                        const uint8_t *B, size_t n, double *out);
+/* FP64 symmetric full (light = 0) / light (light = 1) estimators, [m, n]. */
+void tqo_distance_m_to_n_symmetric_full_f64(const tqo_space *s, const uint8_t *A, size_t m,
+                                            const uint8_t *B, size_t n, int light,
+                                            double *out);
 
 #ifdef __cplusplus
 }

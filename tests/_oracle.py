@@ -59,6 +59,7 @@ def olib():
         L.tqo_distance_symmetric_full_prepared.argtypes = [P, P, P]
         L.tqo_distance_m_to_n_f64.argtypes = [P, P, SZ, P, SZ, P]
         L.tqo_distance_m_to_n_symmetric_f64.argtypes = [P, P, SZ, P, SZ, P]
+        L.tqo_distance_m_to_n_symmetric_full_f64.argtypes = [P, P, SZ, P, SZ, ctypes.c_int, P]
         _o = L
     return _o
 
@@ -144,6 +145,12 @@ class Oracle:
     def symmetric_f64(self, A, B):
         out = np.zeros((len(A), len(B)), np.float64)
         olib().tqo_distance_m_to_n_symmetric_f64(self.h, ptr(A), len(A), ptr(B), len(B), ptr(out))
+        return out
+
+    def symmetric_full_f64(self, A, B, light=False):
+        out = np.zeros((len(A), len(B)), np.float64)
+        olib().tqo_distance_m_to_n_symmetric_full_f64(self.h, ptr(A), len(A), ptr(B), len(B),
+                                                      1 if light else 0, ptr(out))
         return out
 
     def trailer(self, codes):

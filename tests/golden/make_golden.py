@@ -34,6 +34,8 @@ SHAPES = [
     (128, 4), (768, 8), (1000, 4), (96, 4), (128, 8), (384, 4), (768, 4), (1536, 4),
     (1, 4), (2, 8), (3, 3), (4, 2), (5, 4), (7, 9), (8, 4), (13, 5), (16, 6), (33, 7),
     (64, 4), (100, 8), (256, 9), (512, 3), (2048, 8),
+    # padded dims above 4096 (the reference pads any dim)
+    (5000, 4), (7000, 8),
 ]
 
 

@@ -35,7 +35,7 @@ def test_bad_arguments_fail_before_device_use():
     assert b"bits_per_coord" in L.tq_last_error()
     assert L.tq_space_create(64, 10, 42, 137, ctypes.byref(h)) == capi.TQ_EINVAL
     assert L.tq_space_create(0, 4, 42, 137, ctypes.byref(h)) == capi.TQ_EINVAL
-    assert L.tq_space_create(5000, 4, 42, 137, ctypes.byref(h)) == capi.TQ_EUNSUPPORTED
+    assert L.tq_space_create(8193, 4, 42, 137, ctypes.byref(h)) == capi.TQ_EUNSUPPORTED
     assert L.tq_space_create(64, 4, 42, 137, None) == capi.TQ_EINVAL
     assert L.tq_encode_batch(None, None, 1, None, 0, None) == capi.TQ_EINVAL
     assert L.tq_device_count() >= 0

@@ -96,12 +96,19 @@ def test_golden_shapes(gpu, torch_cuda, dim, bits):
     A = codes[-4:]
     s = gpu_symmetric(torch, sp, A, codes)
     na = o.trailer(A)[:, 0].astype(np.float64) ** 2
-    assert scale_rel_err(s, GOLD[f"sym_{tag}"], na, xns).max() <= FP32_TOL
+    # Past padded dim 4096 the reference's own fp32 rounding of same-sign sums
+    # (self-distances) exceeds 1e-5 of scale: the bar is then on the excess
+    # over it, measured against the fp64 estimator.
+    excess = fp32_parity_excess(s, GOLD[f"sym_{tag}"], o.symmetric_f64(A, codes), na, xns)
+    assert excess.max() <= FP32_TOL
+    if sp.D <= 4096:
+        assert scale_rel_err(s, GOLD[f"sym_{tag}"], na, xns).max() <= FP32_TOL
     sp.close()
 
 
 @pytest.mark.parametrize("dim,bits", [(128, 4), (768, 4), (768, 8), (1000, 4), (96, 4),
-                                      (384, 4), (1536, 4), (4096, 8), (2048, 2), (17, 9)])
+                                      (384, 4), (1536, 4), (4096, 8), (2048, 2), (17, 9),
+                                      (5000, 4), (8192, 8), (6000, 3)])
 def test_random_encode_and_scan(gpu, torch_cuda, dim, bits):
     say(f"[gpu] random dim={dim} bits={bits}")
     capi, torch = gpu, torch_cuda
@@ -214,7 +221,8 @@ def test_invalid_arguments(gpu):
 
 @pytest.mark.parametrize("dim,bits,m,n", [(768, 4, 70, 1001), (768, 8, 64, 512), (1000, 4, 33, 777),
                                           (128, 4, 130, 300), (96, 4, 5, 257), (1536, 4, 65, 100),
-                                          (32, 9, 3, 50), (4096, 8, 17, 129), (384, 2, 1, 1)])
+                                          (32, 9, 3, 50), (4096, 8, 17, 129), (384, 2, 1, 1),
+                                          (5000, 4, 40, 300), (7000, 8, 20, 100)])
 def test_mfma_against_fp64_estimator(gpu, torch_cuda, dim, bits, m, n):
     say(f"[gpu] mfma dim={dim} bits={bits} m={m} n={n}")
     capi, torch = gpu, torch_cuda

@@ -11,7 +11,7 @@ import json
 import numpy as np
 import pytest
 
-from _oracle import (MANIFEST, Oracle, load_golden, olib, ptr, ref_available, rlib,
+from _oracle import (MANIFEST, Oracle, fp32_parity_excess, load_golden, olib, ptr, ref_available, rlib,
                      scale_rel_err)
 
 GOLD = load_golden()
@@ -198,7 +198,20 @@ def test_distances_match_reference_fixture(dim, bits):
     assert scale_rel_err(got[:1], GOLD[f"d1_{tag}"][None], qns[:1], xns).max() <= 1e-5
     A = codes[-4:]
     na = o.trailer(A)[:, 0].astype(np.float64) ** 2
-    assert scale_rel_err(o.symmetric(A, codes), GOLD[f"sym_{tag}"], na, xns).max() <= 1e-5
+    sym, gold = o.symmetric(A, codes), GOLD[f"sym_{tag}"]
+    if o.D <= 4096:
+        assert scale_rel_err(sym, gold, na, xns).max() <= 1e-5
+    else:
+        # A self-distance at padded dim 8192 sums 8,192 same-sign fp32 terms;
+        # the reference build reassociates that sum (-ffast-math), the oracle
+        # keeps the source's order. Every entry is within 1e-5 of scale of the
+        # reference beyond the reference's own error, or within the textbook
+        # bound of a recursive fp32 sum of D non-negative terms,
+        # (D - 1) * 2^-24 * sum, of the exact (fp64) estimator.
+        exact = o.symmetric_f64(A, codes)
+        excess = fp32_parity_excess(sym, gold, exact, na, xns)
+        bound = (o.D - 1) * 2.0 ** -24 * (na[:, None] + xns[None, :])
+        assert np.all((excess <= 1e-5) | (np.abs(sym - exact) <= bound))
 
 
 @pytest.mark.parametrize("dim,bits", [(128, 4), (768, 8), (1000, 4), (512, 3), (33, 7)])
@@ -235,6 +248,26 @@ def test_fp32_estimator_close_to_fp64_restatement(dim, bits):
     assert scale_rel_err(d32, d64, qns, xns).max() <= 1e-5
     s32, s64 = o.symmetric(codes[:7], codes), o.symmetric_f64(codes[:7], codes)
     assert scale_rel_err(s32, s64, xns[:7], xns).max() <= 1e-5
+
+
+@pytest.mark.parametrize("dim,bits", [(128, 4), (768, 8), (1000, 4), (512, 3), (33, 7)])
+def test_fp64_symmetric_full_light_pinned_to_reference_fixture(dim, bits):
+    # The FP64 restatements of the QJL-corrected symmetric estimators against
+    # the reference build's fp32 values (full and its prepared form, which is
+    # the same quantity, and light). A planted mistake (correction terms
+    # dropped, i.e. the MSE-only estimator) must miss the full fixture.
+    o = Oracle(dim, bits)
+    tag = f"{dim}_{bits}"
+    codes = GOLD[f"codes_{tag}"]
+    A = codes[-4:]
+    na = o.trailer(A)[:, 0].astype(np.float64) ** 2
+    xns = o.trailer(codes)[:, 0].astype(np.float64) ** 2
+    full, light = o.symmetric_full_f64(A, codes), o.symmetric_full_f64(A, codes, light=True)
+    assert scale_rel_err(GOLD[f"symfull_{tag}"], full, na, xns).max() <= 1e-5
+    assert scale_rel_err(GOLD[f"symfullprep_{tag}"], full, na, xns).max() <= 1e-5
+    assert scale_rel_err(GOLD[f"symlight_{tag}"], light, na, xns).max() <= 1e-5
+    mse_only = o.symmetric_f64(A, codes)
+    assert scale_rel_err(GOLD[f"symfull_{tag}"], mse_only, na, xns).max() > 1e-4
 
 
 def test_estimator_approximates_l2():

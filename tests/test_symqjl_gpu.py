@@ -13,7 +13,7 @@ import sys
 import numpy as np
 import pytest
 
-from _oracle import MANIFEST, Oracle, load_golden, olib, ptr, scale_rel_err
+from _oracle import MANIFEST, Oracle, fp32_parity_excess, load_golden, olib, ptr, scale_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -54,13 +54,20 @@ def test_golden_full_light_prepared_rotated(gpu, dim, bits):
     xns = o.trailer(codes)[:, 0].astype(np.float64) ** 2
     full = host_sym(sp, A, codes, capi.TQ_SYM_FULL)
     light = host_sym(sp, A, codes, capi.TQ_SYM_LIGHT)
-    assert scale_rel_err(full, GOLD[f"symfull_{tag}"], na, xns).max() <= FP32_TOL
-    assert scale_rel_err(light, GOLD[f"symlight_{tag}"], na, xns).max() <= FP32_TOL
     prep = np.zeros((4, sp.sym_prep_stride), np.float32)
     sp.prepare_symmetric(A, 4, prep, mem=capi.TQ_MEM_HOST)
     fp = np.zeros((4, len(codes)), np.float32)
     sp.symmetric_prepared(prep, 4, codes, len(codes), fp, mem=capi.TQ_MEM_HOST)
-    assert scale_rel_err(fp, GOLD[f"symfullprep_{tag}"], na, xns).max() <= FP32_TOL
+    ex_full, ex_light = o.symmetric_full_f64(A, codes), o.symmetric_full_f64(A, codes, True)
+    for got, key, exact in ((full, "symfull", ex_full), (light, "symlight", ex_light),
+                            (fp, "symfullprep", ex_full)):
+        # Past padded dim 4096 the reference's own rounding of same-sign sums
+        # exceeds 1e-5 of scale: the bar is then on the excess over it,
+        # against the FP64 restatement.
+        excess = fp32_parity_excess(got, GOLD[f"{key}_{tag}"], exact, na, xns)
+        assert excess.max() <= FP32_TOL, key
+        if sp.D <= 4096:
+            assert scale_rel_err(got, GOLD[f"{key}_{tag}"], na, xns).max() <= FP32_TOL, key
     X = np.ascontiguousarray(GOLD[f"X_{tag}"])
     rot = np.zeros((len(X), sp.D), np.float32)
     sp.rotated(X, len(X), rot, mem=capi.TQ_MEM_HOST)

@@ -151,7 +151,7 @@ int tq_space_create(uint64_t dim, int bits, uint64_t rot_seed, uint64_t qjl_seed
   const uint64_t want = dim < 4 ? 4 : dim;
   while (D < want) D <<= 1;
   if (!tq::encode_supported((int)D) || D > (1ULL << 20))
-    return fail(TQ_EUNSUPPORTED, "TurboQuantSpace: padded dim %llu is above the GPU limit 4096",
+    return fail(TQ_EUNSUPPORTED, "TurboQuantSpace: padded dim %llu is above the GPU limit 8192",
                 (unsigned long long)D);
   tq_space_s *s = new (std::nothrow) tq_space_s();
   if (!s) return fail(TQ_ENOMEM, "tq_space_create: host allocation failed");

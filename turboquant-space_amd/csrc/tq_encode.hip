@@ -505,12 +505,14 @@ hipError_t launch_g(const SpaceDev &s, const float *X, long n, uint8_t *codes, f
   const long per_block = (threads / 64) * (64 / G);
   const long blocks = (n + per_block - 1) / per_block;
   const size_t lds = sizeof(float) * (2 * (size_t)s.D + 2 * (size_t)s.L);
-  if (s.nib)
-    hipLaunchKernelGGL((k_encode<G, TP, true, MODE>), dim3((unsigned)blocks), dim3(threads), lds,
-                       st, s, X, n, codes, prep);
-  else
-    hipLaunchKernelGGL((k_encode<G, TP, false, MODE>), dim3((unsigned)blocks), dim3(threads), lds,
-                       st, s, X, n, codes, prep);
+  // Past 64 KB (padded dim 8192) the kernel asks for more dynamic LDS.
+  auto kern = s.nib ? k_encode<G, TP, true, MODE> : k_encode<G, TP, false, MODE>;
+  if (lds > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(threads), lds, st, s, X, n, codes, prep);
   return hipGetLastError();
 }
 
@@ -535,13 +537,14 @@ hipError_t dispatch(const SpaceDev &s, const float *X, long n, uint8_t *codes, f
     case 1024: return launch_g<16, 64, MODE>(s, X, n, codes, prep, st);
     case 2048: return launch_g<32, 64, MODE>(s, X, n, codes, prep, st);
     case 4096: return launch_g<64, 64, MODE>(s, X, n, codes, prep, st);
+    case 8192: return launch_g<64, 128, MODE>(s, X, n, codes, prep, st);
     default: return hipErrorInvalidValue;
   }
 }
 
 }  // namespace
 
-bool encode_supported(int D) { return D >= 4 && D <= 4096; }
+bool encode_supported(int D) { return D >= 4 && D <= 8192; }
 
 hipError_t launch_encode(const SpaceDev &s, const float *X, long n, uint8_t *codes,
                          hipStream_t st) {

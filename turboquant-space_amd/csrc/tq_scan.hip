@@ -170,7 +170,7 @@ hipError_t launch_cpl(const SpaceDev &s, const float *prep, long m, const uint8_
 
 }  // namespace
 
-bool scan_supported(int D) { return D >= 4 && D <= 4096; }
+bool scan_supported(int D) { return D >= 4 && D <= 8192; }
 
 hipError_t launch_scan_fp32(const SpaceDev &s, const float *prep, long m, const uint8_t *codes,
                             long n, float *out, bool symmetric, hipStream_t st, void *ws) {
@@ -188,6 +188,7 @@ hipError_t launch_scan_fp32(const SpaceDev &s, const float *prep, long m, const 
     case 1024: return launch_cpl<16>(s, prep, m, codes, n, out, symmetric, st);
     case 2048: return launch_cpl<32>(s, prep, m, codes, n, out, symmetric, st);
     case 4096: return launch_cpl<64>(s, prep, m, codes, n, out, symmetric, st);
+    case 8192: return launch_cpl<128>(s, prep, m, codes, n, out, symmetric, st);
     default: return hipErrorInvalidValue;
   }
 }

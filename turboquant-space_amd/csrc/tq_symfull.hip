@@ -132,6 +132,12 @@ __global__ __launch_bounds__(256) void k_sym_qjl(SpaceDev s, const float *__rest
           aq += (uu & 1u) ? l_h[i] : -l_h[i];
         }
       }
+      // Eight partial sums per term (one per coordinate residue mod 8),
+      // folded pairwise at the end: same-sign sums over large D (self-
+      // distances) keep ~1/8 of a single chain's rounding.
+      float pm[8], pg[8], pq[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) pm[k] = pg[k] = pq[k] = 0.0f;
       for (int w = 0; w < D / 8; ++w) {
         uint32_t u[8];
         if constexpr (NIB) {
@@ -147,11 +153,22 @@ __global__ __launch_bounds__(256) void k_sym_qjl(SpaceDev s, const float *__rest
         for (int k = 0; k < 8; ++k) {
           const int i = 8 * w + k;
           const float c = l_cent[u[k] >> 1];
-          am = fmaf(l_r[i], c, am);
-          ag = fmaf(l_g[i], c, ag);
-          aq += (u[k] & 1u) ? l_h[i] : -l_h[i];
+          pm[k] = fmaf(l_r[i], c, pm[k]);
+          pg[k] = fmaf(l_g[i], c, pg[k]);
+          pq[k] += (u[k] & 1u) ? l_h[i] : -l_h[i];
         }
       }
+#pragma unroll
+      for (int h = 4; h >= 1; h >>= 1)
+#pragma unroll
+        for (int k = 0; k < h; ++k) {
+          pm[k] += pm[k + h];
+          pg[k] += pg[k + h];
+          pq[k] += pq[k + h];
+        }
+      am += pm[0];
+      ag += pg[0];
+      aq += pq[0];
       const float xn = load_f32(cp + s.packed, al);
       const float gamma = load_f32(cp + s.packed + 4, al);
       const float sigma = load_f32(cp + s.packed + 8, al);
@@ -168,7 +185,7 @@ __global__ __launch_bounds__(256) void k_sym_qjl(SpaceDev s, const float *__rest
 
 }  // namespace
 
-bool sym_qjl_supported(const SpaceDev &s) { return s.D >= 4 && s.D <= 4096; }
+bool sym_qjl_supported(const SpaceDev &s) { return s.D >= 4 && s.D <= 8192; }
 
 hipError_t launch_prepare_sym(const SpaceDev &s, const uint8_t *codes, long m, bool light,
                               float *sp, hipStream_t st) {
@@ -188,12 +205,14 @@ hipError_t launch_sym_qjl(const SpaceDev &s, const float *sp, long m, const uint
   if (bx > 1024) bx = 1024;
   const long by = m < 65535 ? m : 65535;
   const size_t lds = sizeof(float) * (3 * (size_t)s.D + 256);
-  if (s.nib)
-    hipLaunchKernelGGL(k_sym_qjl<true>, dim3((unsigned)bx, (unsigned)by), dim3(256), lds, st, s, sp,
-                       m, codes, n, out);
-  else
-    hipLaunchKernelGGL(k_sym_qjl<false>, dim3((unsigned)bx, (unsigned)by), dim3(256), lds, st, s, sp,
-                       m, codes, n, out);
+  auto kern = s.nib ? k_sym_qjl<true> : k_sym_qjl<false>;
+  if (lds > 65536) {  // padded dim 8192: more than the default dynamic LDS
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)bx, (unsigned)by), dim3(256), lds, st, s, sp, m, codes,
+                     n, out);
   return hipGetLastError();
 }
 
