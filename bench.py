@@ -192,28 +192,46 @@ SERVE_COLLS = 4
 SERVE_N = 10_000_000
 
 
+SERVE_SLO_MS = 50.0                       # p99 bar of the load curves' SLO figure
+SERVE_QPS = {"mfma": [4, 25, 50, 75, 100, 125, 150, 200], "fp32": [0.5, 1, 2, 4, 8, 12]}
+
+
 def bench_serving(torch, capi, sh, qps_by_mode, seconds, n_per_coll=SERVE_N, seed=21):
-    """Poisson arrivals at a fixed rate; each request a top-10 search of
-    m ~ floor(exp(U(0, ln 257))) queries against a uniformly chosen
-    collection, or with probability 0.05 an append of 10,000 vectors. Latency
-    is submission to results-in-host-memory, measured by the server."""
+    """Load curves of the request server, one per form (MFMA, plain fp32):
+    Poisson arrivals at each offered rate of `qps_by_mode` for `seconds`
+    each; a request is a top-10 search of m ~ floor(exp(U(0, ln 257))) queries
+    against a uniformly chosen collection, or with probability 0.05 an append
+    of 10,000 vectors. The 4 collections are consecutive 10M-row slices of one
+    Gaussian mixture over 40M vectors (8,192 centres, std 0.3, normalised,
+    seed 20). Latency is submission to results-in-host-memory, measured by
+    the server. A sweep stops past saturation (achieved QPS below 90% of the
+    realized arrival rate, so the backlog grows for as long as the point
+    runs). `slo_qps` is the highest offered rate served at p99 <=
+    SERVE_SLO_MS with at least 95% of the realized rate achieved."""
     import threading
     results = {}
     sp = capi.Space(DIM, BITS)
-    # Collections are generated once on the device and loaded into each server.
-    staged = []
-    for c in range(SERVE_COLLS):
-        codes = torch.empty((n_per_coll, sp.cs), dtype=torch.uint8, device="cuda")
-        centres = None
-        for start, x, centres in gaussian_mixture(torch, n_per_coll, DIM, 8192, 0.3, 20 + 100 * c):
-            sp.encode(x, x.shape[0], codes[start:start + x.shape[0]], stream=sh)
-        staged.append((codes, centres))
+    # The 40M-vector mixture, encoded once on the device, one buffer per slice.
+    staged = [torch.empty((n_per_coll, sp.cs), dtype=torch.uint8, device="cuda")
+              for _ in range(SERVE_COLLS)]
+    centres = None
+    for start, x, centres in gaussian_mixture(torch, SERVE_COLLS * n_per_coll, DIM, 8192, 0.3, 20):
+        c, off = divmod(start, n_per_coll)
+        rows = x.shape[0]
+        while rows:  # a generator chunk may straddle two slices
+            take = min(rows, n_per_coll - off)
+            sp.encode(x[x.shape[0] - rows:x.shape[0] - rows + take].contiguous(), take,
+                      staged[c][off:off + take], stream=sh)
+            rows -= take
+            c, off = c + 1, 0
     torch.cuda.synchronize()
     rng = np.random.default_rng(seed)
-    horizon = max(seconds * max(qps_by_mode.values()) * 1.5, 50)
-    # Request pool, drawn once: queries and append payloads on the host.
+    longest = max(seconds * max(q) for q in qps_by_mode.values())
+    horizon = int(max(longest * 1.3, 50))
+    # Request pool, drawn once: kinds and sizes; search queries are drawn per
+    # request, append payloads from a pool of 16 (host memory stays bounded).
     reqs = []
-    for i in range(int(horizon)):
+    for i in range(horizon):
         c = int(rng.integers(0, SERVE_COLLS))
         if rng.random() < 0.05:
             reqs.append(("append", c, 10_000))
@@ -221,23 +239,21 @@ def bench_serving(torch, capi, sh, qps_by_mode, seconds, n_per_coll=SERVE_N, see
             m = int(min(np.floor(np.exp(rng.uniform(0.0, np.log(257)))), 256))
             reqs.append(("search", c, m))
     payload = {}
+    appends = []
+    for i in range(16):
+        _, x, _ = next(gaussian_mixture(torch, 10_000, DIM, 8192, 0.3, 4000 + i, centres=centres))
+        appends.append(np.ascontiguousarray(x.cpu().numpy()))
     for i, (kind, c, rows) in enumerate(reqs):
-        _, x, _ = next(gaussian_mixture(torch, rows, DIM, 8192, 0.3, 5000 + i,
-                                        centres=staged[c][1]))
+        if kind == "append":
+            payload[i] = appends[i % 16]
+            continue
+        _, x, _ = next(gaussian_mixture(torch, rows, DIM, 8192, 0.3, 5000 + i, centres=centres))
         payload[i] = np.ascontiguousarray(x.cpu().numpy())
-    for mode_name, qps in qps_by_mode.items():
-        mode = capi.TQ_MODE_MFMA if mode_name == "mfma" else capi.TQ_MODE_FP32
-        cap = n_per_coll + 20 * 10_000 * 2
-        srv = capi.Server(mode=mode, k=10, hbm_budget=96 << 30, slot_bytes=2 << 30,
-                          max_inflight=2, max_batch_queries=4096)
-        ids = []
-        for c in range(SERVE_COLLS):
-            ids.append(srv.add_collection(sp, cap))
-            srv.load_codes(ids[-1], staged[c][0], n_per_coll)
+
+    def run_point(srv, ids, qps):
         arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / qps, len(reqs)))
         count = int(np.searchsorted(arr, seconds))
         tickets = [None] * count
-        outs = {}
         done = []
         lock = threading.Condition()
 
@@ -249,6 +265,7 @@ def bench_serving(torch, capi, sh, qps_by_mode, seconds, n_per_coll=SERVE_N, see
                 lat = srv.wait(tickets[i])
                 done.append((i, lat, time.perf_counter()))
 
+        st0 = srv.stats()
         th = threading.Thread(target=waiter)
         th.start()
         t_start = time.perf_counter()
@@ -262,7 +279,6 @@ def bench_serving(torch, capi, sh, qps_by_mode, seconds, n_per_coll=SERVE_N, see
             else:
                 od = np.empty((rows, 10), np.float32)
                 oi = np.empty((rows, 10), np.int64)
-                outs[i] = (od, oi)
                 t = srv.search(ids[c], payload[i], od, oi)
             with lock:
                 tickets[i] = t
@@ -270,29 +286,59 @@ def bench_serving(torch, capi, sh, qps_by_mode, seconds, n_per_coll=SERVE_N, see
         th.join()
         t_end = max(d[2] for d in done) if done else time.perf_counter()
         st = srv.stats()
-        srv.close()
         lat_s = np.array([d[1] for d in done if reqs[d[0]][0] == "search"])
         lat_a = np.array([d[1] for d in done if reqs[d[0]][0] == "append"])
         nq = sum(reqs[i][2] for i in range(count) if reqs[i][0] == "search")
         wall = t_end - t_start
-        results[mode_name] = {
-            "qps_offered": qps, "requests": count, "searches": int(len(lat_s)),
-            "appends": int(len(lat_a)), "seconds": wall,
-            "qps_achieved": count / wall, "queries_per_s": nq / wall,
-            "dist_per_s": nq * n_per_coll / wall,
+        return {
+            "qps_offered": qps, "qps_realized": count / seconds, "requests": count,
+            "searches": int(len(lat_s)), "appends": int(len(lat_a)), "seconds": wall,
+            "qps_achieved": count / wall,
+            "queries_per_s": nq / wall, "dist_per_s": nq * n_per_coll / wall,
             "p50_ms": float(np.percentile(lat_s, 50) * 1e3),
             "p99_ms": float(np.percentile(lat_s, 99) * 1e3),
             "mean_ms": float(lat_s.mean() * 1e3),
             "append_p50_ms": float(np.percentile(lat_a, 50) * 1e3) if len(lat_a) else None,
-            "batches": st["batches"], "max_batch_requests": st["max_batch_requests"],
-            "failed": st["failed"]}
-        log(f"serving {mode_name} @ {qps} QPS: {count} requests, p50 "
-            f"{results[mode_name]['p50_ms']:.1f} ms, p99 {results[mode_name]['p99_ms']:.1f} ms, "
-            f"{st['batches']} batches")
+            "batches": st["batches"] - st0["batches"],
+            "requests_per_batch": count / max(1, st["batches"] - st0["batches"]),
+            "max_batch_requests": st["max_batch_requests"],
+            "failed": st["failed"] - st0["failed"]}
+
+    for mode_name, points in qps_by_mode.items():
+        mode = capi.TQ_MODE_MFMA if mode_name == "mfma" else capi.TQ_MODE_FP32
+        cap = n_per_coll + 2_000_000
+        srv = capi.Server(mode=mode, k=10, hbm_budget=96 << 30, slot_bytes=2 << 30,
+                          max_inflight=2, max_batch_queries=4096)
+        ids = []
+        for c in range(SERVE_COLLS):
+            ids.append(srv.add_collection(sp, cap))
+            srv.load_codes(ids[-1], staged[c], n_per_coll)
+        curve = []
+        for qps in points:
+            pt = run_point(srv, ids, qps)
+            curve.append(pt)
+            log(f"serving {mode_name} @ {qps} QPS: {pt['requests']} requests, achieved "
+                f"{pt['qps_achieved']:.1f} QPS, p50 {pt['p50_ms']:.1f} ms, p99 {pt['p99_ms']:.1f} ms, "
+                f"{pt['requests_per_batch']:.2f} requests/batch")
+            if pt["qps_achieved"] < 0.9 * pt["qps_realized"]:
+                break  # past saturation: the backlog grows for as long as the point runs
+        srv.close()
+        ok = [pt["qps_offered"] for pt in curve
+              if pt["p99_ms"] <= SERVE_SLO_MS and pt["qps_achieved"] >= 0.95 * pt["qps_realized"]]
+        last = curve[-1]
+        results[mode_name] = {"curve": curve, "slo_p99_ms": SERVE_SLO_MS,
+                              "slo_qps": max(ok) if ok else None,
+                              "saturated_at_qps": last["qps_offered"]
+                              if last["qps_achieved"] < 0.9 * last["qps_realized"] else None,
+                              "peak_queries_per_s": max(pt["queries_per_s"] for pt in curve)}
+        log(f"serving {mode_name}: highest QPS at p99 <= {SERVE_SLO_MS:.0f} ms: "
+            f"{results[mode_name]['slo_qps']}")
     del staged
     torch.cuda.empty_cache()
     return {"collections": SERVE_COLLS, "codes_per_collection": n_per_coll, "dim": DIM,
-            "bits": BITS, "k": 10, "modes": results}
+            "bits": BITS, "k": 10, "seconds_per_point": seconds,
+            "data": "one Gaussian mixture over 40M vectors (8,192 centres, std 0.3, normalised, "
+                    "seed 20), 4 consecutive slices", "modes": results}
 
 
 RECALL_N = 1_000_000
@@ -431,43 +477,117 @@ def bench_recall(torch, capi, sh, n=RECALL_N, m=RECALL_Q, k=100):
     return res
 
 
-def cpu_baseline(codes_host, Q_host, budget_s=8.0):
-    """The reference CPU build (oracle/_ref, compiled in place from the
-    reference sources with its Release flags) on this host's cores:
-    distance_m_to_n with m = 128 (> 64, so its OpenMP loop engages) over a
-    prefix of the collection sized to the time budget; min of 3 runs."""
-    lib_path = os.path.join(REPO, "oracle", "_ref", "libtqref.so")
-    if not os.path.exists(lib_path):
-        return None
-    L = ctypes.CDLL(lib_path)
+def cpu_info():
+    """The host the CPU baseline runs on: CPU model, logical CPUs, the CPUs this
+    process may run on, and physical cores (/proc/cpuinfo)."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                    cores.add((phys, core))
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "usable_cpus": len(os.sched_getaffinity(0)), "physical_cores": len(cores) or None}
+
+
+def cpu_child(path):
+    """The CPU baseline's child process (no torch, so OpenMP starts with the
+    environment the parent set): the reference build on the sample in `path`,
+    three legs, min of 3 runs each; prints one JSON line."""
+    d = np.load(path)
+    codes, Q, X = d["codes"], np.ascontiguousarray(d["Q"]), np.ascontiguousarray(d["X"])
+    L = ctypes.CDLL(os.path.join(REPO, "oracle", "_ref", "libtqref.so"))
     P, SZ = ctypes.c_void_p, ctypes.c_size_t
     L.tqref_create.restype = P
     L.tqref_create.argtypes = [SZ, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int]
     L.tqref_num_threads.restype = ctypes.c_int
     L.tqref_num_threads.argtypes = [P]
     L.tqref_distance_m_to_n.argtypes = [P, P, SZ, P, SZ, P]
+    L.tqref_distance_1_to_n.argtypes = [P, P, P, SZ, P]
+    L.tqref_encode_batch.argtypes = [P, P, SZ, P]
     L.tqref_destroy.argtypes = [P]
     h = L.tqref_create(DIM, BITS, 42, 137, 0)
     threads = L.tqref_num_threads(h)
+
+    def best_of(fn, reps=3):
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    # m x n with m = 128 (> 64, so the reference's OpenMP loop over m
+    # engages), n sized to ~8 s over the three runs.
     m = 128
-    Q = np.ascontiguousarray(Q_host[:m])
-    n = 20_000
+    out = np.zeros((m, 20_000), np.float32)
+    t = best_of(lambda: L.tqref_distance_m_to_n(h, Q.ctypes.data, m, codes.ctypes.data, 20_000,
+                                                out.ctypes.data), 1)
+    n = int(min(len(codes), max(20_000, 8.0 / 3 / (t / (m * 20_000)) / m)))
     out = np.zeros((m, n), np.float32)
-    t0 = time.perf_counter()
-    L.tqref_distance_m_to_n(h, Q.ctypes.data, m, codes_host.ctypes.data, n, out.ctypes.data)
-    per = (time.perf_counter() - t0) / (m * n)
-    n = int(min(len(codes_host), max(20_000, budget_s / 3 / per / m)))
-    out = np.zeros((m, n), np.float32)
-    best = float("inf")
-    for _ in range(3):
-        t0 = time.perf_counter()
-        L.tqref_distance_m_to_n(h, Q.ctypes.data, m, codes_host.ctypes.data, n, out.ctypes.data)
-        best = min(best, time.perf_counter() - t0)
+    t_mn = best_of(lambda: L.tqref_distance_m_to_n(h, Q.ctypes.data, m, codes.ctypes.data, n,
+                                                   out.ctypes.data))
+    # distance_1_to_n over every sampled code (OpenMP over n).
+    n1 = len(codes)
+    one = np.zeros(n1, np.float32)
+    t_1n = best_of(lambda: L.tqref_distance_1_to_n(h, Q.ctypes.data, codes.ctypes.data, n1,
+                                                   one.ctypes.data), 5)
+    # encode_batch of the raw sample (OpenMP over rows).
+    cbuf = np.zeros((len(X), codes.shape[1]), np.uint8)
+    t_enc = best_of(lambda: L.tqref_encode_batch(h, X.ctypes.data, len(X), cbuf.ctypes.data))
     L.tqref_destroy(h)
-    return {"value": m * n / best, "unit": "distances/s", "cores": threads,
+    print(json.dumps({"threads": threads, "m": m, "n": n, "t_mn": t_mn, "n1": n1, "t_1n": t_1n,
+                      "n_enc": len(X), "t_enc": t_enc}), flush=True)
+    return 0
+
+
+def cpu_baseline(codes_host, Q_host, X_host):
+    """The reference CPU build (oracle/_ref, compiled in place from the
+    reference sources with its Release flags) on this host's cores, in a
+    child process with OMP_PROC_BIND=close and OMP_PLACES=cores (threads:
+    OMP_NUM_THREADS as set, else every usable CPU): distance_m_to_n
+    (m = 128 over a prefix of the collection sized to ~8 s), distance_1_to_n
+    over the whole sample and encode_batch of raw vectors of the same recipe;
+    min of 3 (1-to-N: 5) runs."""
+    import subprocess
+    import tempfile
+    if not os.path.exists(os.path.join(REPO, "oracle", "_ref", "libtqref.so")):
+        return None
+    info = cpu_info()
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+    env.setdefault("OMP_NUM_THREADS", str(info["usable_cpus"]))
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "cpu_sample.npz")
+        np.savez(path, codes=codes_host, Q=Q_host, X=X_host)
+        p = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", path],
+                           stdout=subprocess.PIPE, env=env, text=True, timeout=600)
+    if p.returncode != 0:
+        log(f"cpu baseline child failed ({p.returncode})")
+        return None
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    info.update({"omp_num_threads": env["OMP_NUM_THREADS"], "omp_proc_bind": "close",
+                 "omp_places": "cores"})
+    return {"value": r["m"] * r["n"] / r["t_mn"], "unit": "distances/s", "cores": r["threads"],
             "kind": "reference",
-            "sample": f"distance_m_to_n, m={m} queries x n={n} codes of the same collection "
-                      f"(dim {DIM}, {BITS}-bit), min of 3 runs, OMP_NUM_THREADS={threads}"}
+            "sample": f"distance_m_to_n, m={r['m']} queries x n={r['n']} codes of the same "
+                      f"collection (dim {DIM}, {BITS}-bit), min of 3 runs",
+            "cpu": info,
+            "legs": {
+                "distance_1_to_n": {"dist_per_s": r["n1"] / r["t_1n"],
+                                    "sample": f"1 query x {r['n1']} codes, min of 5 runs"},
+                "encode": {"vec_per_s": r["n_enc"] / r["t_enc"],
+                           "sample": f"encode_batch of {r['n_enc']} vectors of the same recipe, "
+                                     "min of 3 runs"}}}
 
 
 def main():
@@ -486,15 +606,21 @@ def main():
                     help="skip the symmetric (dim 1000, 4,096 x 2M) measurement")
     ap.add_argument("--no-asym8", action="store_true",
                     help="skip the 8-bit (dim 768, 1,024 x 1M) measurement")
-    ap.add_argument("--serve-seconds", type=float, default=12.0)
-    ap.add_argument("--qps-mfma", type=float, default=40.0)
-    ap.add_argument("--qps-fp32", type=float, default=4.0)
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="seconds per offered-load point of the serving curves")
+    ap.add_argument("--qps-mfma", type=str, default=",".join(str(q) for q in SERVE_QPS["mfma"]),
+                    help="offered QPS points of the MFMA serving curve (comma-separated)")
+    ap.add_argument("--qps-fp32", type=str, default=",".join(str(q) for q in SERVE_QPS["fp32"]),
+                    help="offered QPS points of the plain fp32 serving curve")
     ap.add_argument("--serve-codes", type=int, default=SERVE_N,
                     help="codes per resident collection (4 collections)")
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--calibrate", action="store_true",
                     help="run one 1 GiB device-to-device copy first (a kernel of known "
                          "traffic, used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.cpu_child:
+        return cpu_child(args.cpu_child)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the distance path runs on a single MI355X)", file=sys.stderr)
@@ -591,9 +717,14 @@ def main():
         n_cpu = min(n, 2_000_000)
         codes_host = codes[:n_cpu].cpu().numpy()
         Q_host = Qd[:128].cpu().numpy()
-        cpu = cpu_baseline(codes_host, Q_host)
+        _, Xs, _ = next(gaussian_mixture(torch, 100_000, DIM, 8192, 0.3, 22, centres=centres))
+        cpu = cpu_baseline(codes_host, Q_host, Xs.cpu().numpy())
+        del Xs
         if cpu:
-            log(f"cpu baseline: {cpu['value'] / 1e6:.2f} M dist/s on {cpu['cores']} threads")
+            log(f"cpu baseline ({cpu['cpu']['model']}, {cpu['cores']} threads): m x n "
+                f"{cpu['value'] / 1e6:.2f} M dist/s, 1-to-N "
+                f"{cpu['legs']['distance_1_to_n']['dist_per_s'] / 1e6:.1f} M dist/s, encode "
+                f"{cpu['legs']['encode']['vec_per_s'] / 1e3:.0f} k vec/s")
 
     # The collection and output of the headline measurement are released
     # before the batched and serving runs build their own.
@@ -605,8 +736,10 @@ def main():
     batched = None if args.no_batched else bench_batched(torch, capi, stream, sh)
     serving = None
     if not args.no_serving:
-        serving = bench_serving(torch, capi, sh, {"mfma": args.qps_mfma, "fp32": args.qps_fp32},
-                                args.serve_seconds, n_per_coll=args.serve_codes)
+        pts = {k: [float(q) for q in v.split(",") if q]
+               for k, v in (("mfma", args.qps_mfma), ("fp32", args.qps_fp32))}
+        serving = bench_serving(torch, capi, sh, pts, args.serve_seconds,
+                                n_per_coll=args.serve_codes)
 
     line = {
         "metric": "distances/s at dim768 4-bit, MFMA vs fp32 GPU path; p99 latency @ fixed QPS",

@@ -224,3 +224,43 @@ def test_add_collections_while_searches_run(gpu, torch_cuda):
         np.testing.assert_array_equal(od.view(np.uint32), pd.view(np.uint32))
     assert srv.stats()["failed"] == 0
     srv.close()
+
+
+def test_wrapper_holds_request_buffers(gpu, torch_cuda):
+    """The native server has only the addresses of a request's host arrays:
+    the Python wrapper must keep them alive until the ticket is waited for,
+    also when the caller drops its own references right after submitting."""
+    import gc
+    import weakref
+    capi = gpu
+    say("[gpu] server wrapper holds request buffers")
+    rng = np.random.default_rng(12)
+    srv, colls = make_server(capi, capi.TQ_MODE_MFMA, SPECS[:1], rng, max_batch_queries=64)
+    c, sp, centres, _ = colls[0]
+    refs, tickets = [], []
+    for _ in range(24):
+        Q = gm(rng, int(rng.integers(1, 48)), sp.dim, centres)
+        od = np.zeros((len(Q), 10), np.float32)
+        oi = np.zeros((len(Q), 10), np.int64)
+        tickets.append(srv.search(c, Q, od, oi))
+        refs.append((weakref.ref(Q), weakref.ref(od), weakref.ref(oi)))
+    X = gm(rng, 500, sp.dim, centres)
+    xref = weakref.ref(X)
+    tickets.append(srv.append(c, X))
+    del Q, od, oi, X
+    gc.collect()
+    assert all(r() is not None for trio in refs for r in trio) and xref() is not None
+    def check(t, trio):
+        # Strong references taken before the wait, which releases the wrapper's.
+        _, od, oi = (r() for r in trio)
+        srv.wait(t)
+        # Results written before the wait returned: every row holds k valid ids.
+        assert (oi >= 0).all() and np.isfinite(od).all()
+
+    for t, trio in zip(tickets, refs):
+        check(t, trio)
+    srv.wait(tickets[-1])
+    gc.collect()
+    assert all(r() is None for trio in refs for r in trio) and xref() is None
+    assert srv.stats()["failed"] == 0
+    srv.close()

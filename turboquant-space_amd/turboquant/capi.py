@@ -355,8 +355,10 @@ def topk_rows(dist, m, n, k, out_dist, out_ids, stream=None):
 
 
 class Server:
-    """Owns a tq_server. Inputs and outputs are host numpy arrays, which the
-    caller keeps alive until the request's ticket is waited for."""
+    """Owns a tq_server. Inputs and outputs are host numpy arrays; the server
+    reads and writes them asynchronously, so this wrapper holds a reference to
+    each request's arrays until its ticket is waited for (the native server
+    only has their addresses)."""
 
     def __init__(self, mode=TQ_MODE_MFMA, k=10, hbm_budget=8 << 30, slot_bytes=0,
                  max_inflight=0, max_batch_queries=0):
@@ -366,6 +368,7 @@ class Server:
         self.h = h
         self.k = k
         self._spaces = []
+        self._held = {}
 
     def add_collection(self, space, capacity):
         c = ctypes.c_uint32()
@@ -376,17 +379,22 @@ class Server:
     def append(self, coll, X):
         t = ctypes.c_uint64()
         check(lib().tq_server_submit_append(self.h, coll, ptr(X), len(X), ctypes.byref(t)))
+        self._held[int(t.value)] = (X,)
         return int(t.value)
 
     def search(self, coll, Q, out_d, out_i):
         t = ctypes.c_uint64()
         check(lib().tq_server_submit_search(self.h, coll, ptr(Q), len(Q), ptr(out_d), ptr(out_i),
                                             ctypes.byref(t)))
+        self._held[int(t.value)] = (Q, out_d, out_i)
         return int(t.value)
 
     def wait(self, ticket):
         lat = ctypes.c_double()
-        check(lib().tq_server_wait(self.h, ticket, ctypes.byref(lat)))
+        try:
+            check(lib().tq_server_wait(self.h, ticket, ctypes.byref(lat)))
+        finally:
+            self._held.pop(int(ticket), None)
         return float(lat.value)
 
     def collection(self, coll):
@@ -412,6 +420,7 @@ class Server:
         if self.h:
             lib().tq_server_destroy(self.h)
             self.h = None
+        self._held.clear()
 
     def __del__(self):
         try:
