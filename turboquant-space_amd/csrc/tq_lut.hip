@@ -28,6 +28,8 @@
 // is the reference's fp32 expression.
 #include "tq_common.h"
 
+#include <type_traits>
+
 namespace tq {
 namespace {
 
@@ -344,6 +346,271 @@ __global__ __launch_bounds__(LutShape<D>::THREADS) void k_lut_nib(SpaceDev s, co
   }
 }
 
+// Wide form, D = 512 and 1024. In the narrow form above the lanes that score
+// the same chunk of different codes read random entries of the same 64-entry
+// table, so a wavefront's lookup lands on a few banks several times over.
+// Here every lane of a group of LPC lanes owns its tables: LPC lanes score
+// one code, lane ll = l % LPC taking the NW code words [NW ll, NW ll + NW)
+// (D = 1024: 32 lanes x 16 bytes, so one load instruction reads two codes;
+// D = 512: 64 lanes x 4 bytes). The tables are laid out so that every entry
+// lane ll reads sits in LDS bank ll (+ 32 for odd words at D = 1024): the
+// entry of indices (lo, hi) in the table of byte kk of word u of lane ll's
+// slice is at byte address
+//   4 ll + 128 (u & 1) + 256 (b & 0xEE) + 256 (kk & 1) + 4096 (kk >> 1) + 65536 (u >> 1),
+// b & 0xEE = 2 lo + 32 hi being the code byte without its QJL bits. Within a
+// half-wavefront no two lanes share a bank (SQ_LDS_BANK_CONFLICT reads 0 on
+// MI355X). One v_perm puts byte kk of w & 0xEEEEEEEE into byte 1 of the
+// lane's base address (the bits 8 and 12 it leaves zero take kk, with u & 1,
+// as the instruction's offset).
+// A wavefront scores 8 consecutive codes per round. Code bytes and trailers
+// are read once, with non-temporal loads (on MI355X a plain 16-byte stream
+// of 5.24 GB reads at 5.9 TB/s, a non-temporal one at 6.3-6.5 TB/s). The 8
+// codes' per-lane partial sums meet through a transposed reduction (permlane
+// swaps, a DPP row rotate, then DPP within 8 lanes), after which lanes
+// 8c .. 8c + 7 hold the sums of one code and lane 8c writes its distance.
+// The tables and QJL digits are the narrow form's, so both give the same
+// terms; only the order of the fp32 MSE additions differs.
+constexpr int kWideCodes = 8;  // codes per wavefront round
+
+template <int D>
+struct WideCfg {
+  static_assert(D == 512 || D == 1024, "wide form: D = 512 or 1024");
+  static constexpr int LPC = D == 1024 ? 32 : 64;   // lanes per code
+  static constexpr int NW = D / 8 / LPC;            // code words per lane
+  static constexpr int CPI = 64 / LPC;              // codes per load instruction
+  static constexpr int NL = kWideCodes / CPI;       // load instructions per round
+  static constexpr int THREADS = D == 1024 ? 1024 : 512;
+  static constexpr int R = 3;                       // ring slots
+  static constexpr size_t lds_bytes = sizeof(float) * 64 * (D / 2);
+};
+
+__device__ __forceinline__ float wide_swap_add32(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ int wide_swap_add32(int a, int b) {
+  const auto r = __builtin_amdgcn_permlane32_swap((unsigned)a, (unsigned)b, false, false);
+  return (int)r[0] + (int)r[1];
+}
+__device__ __forceinline__ float wide_swap_add16(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ int wide_swap_add16(int a, int b) {
+  const auto r = __builtin_amdgcn_permlane16_swap((unsigned)a, (unsigned)b, false, false);
+  return (int)r[0] + (int)r[1];
+}
+// x + (lane ^ 8's y), lane ^ 8 within a row of 16 being DPP row_ror:8.
+__device__ __forceinline__ float wide_add_ror8(float x, float y) {
+  return x + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(y), 0x128, 0xF, 0xF, true));
+}
+__device__ __forceinline__ int wide_add_ror8(int x, int y) {
+  return x + __builtin_amdgcn_mov_dpp(y, 0x128, 0xF, 0xF, true);
+}
+__device__ __forceinline__ float wide_group8(float v) { return group_sum_f<8>(v); }
+__device__ __forceinline__ int wide_group8(int v) { return group_sum_i<8>(v); }
+
+// Sums over the lanes of each code of the round's per-lane values v[i] (code
+// CPI i + l / LPC); the result in lane l is the total of code
+// wide_code(l), the same in the 8 lanes 8c .. 8c + 7. Each step halves the
+// values per lane: a permlane swap pairs value i with value i + half, lanes
+// of one half (of the wavefront, or of each 32-lane row pair) keeping the
+// first and the others the second, each summed over both halves; a row
+// rotate by 8 does the same for lane bit 3; then the 8 lanes of a code add
+// up (DPP xor 1, xor 2, half mirror).
+template <int LPC, typename T>
+__device__ __forceinline__ T wide_reduce(const T (&v)[kWideCodes / (64 / LPC)], int lane) {
+  constexpr int NL = kWideCodes / (64 / LPC);
+  T t[2];
+  if constexpr (NL == 8) {  // 64 lanes per code: halves of the wavefront, then of 32-lane rows
+    T s[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[i] = wide_swap_add32(v[i], v[i + 4]);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) t[i] = wide_swap_add16(s[i], s[i + 2]);
+  } else {  // 32 lanes per code: halves of each 32-lane row pair
+    static_assert(NL == 4, "values per lane");
+#pragma unroll
+    for (int i = 0; i < 2; ++i) t[i] = wide_swap_add16(v[i], v[i + 2]);
+  }
+  const bool b3 = (lane >> 3) & 1;
+  return wide_group8(wide_add_ror8(b3 ? t[1] : t[0], b3 ? t[0] : t[1]));
+}
+// The code (0 .. 7 within the round) whose sums lane l holds after wide_reduce.
+template <int LPC>
+__device__ __forceinline__ int wide_code(int lane) {
+  if constexpr (LPC == 64) return (lane >> 3) & 7;
+  else return 2 * ((lane >> 3) & 3) + (lane >> 5);
+}
+
+template <bool SYM, int D>
+__global__ __launch_bounds__(WideCfg<D>::THREADS) void k_lut_wide(
+    SpaceDev s, const float *__restrict__ prep, const int32_t *__restrict__ ql, long m,
+    const uint8_t *__restrict__ codes, long n, float *__restrict__ out) {
+  using Cf = WideCfg<D>;
+  constexpr int LPC = Cf::LPC, NW = Cf::NW, CPI = Cf::CPI, NL = Cf::NL, R = Cf::R;
+  constexpr int THREADS = Cf::THREADS, WAVES = THREADS / 64, CPR = kWideCodes;
+  __shared__ float tab[(D / 2) * 64];
+  typedef uint32_t slice_t __attribute__((ext_vector_type(NW), aligned(4)));
+  typedef float trail_t __attribute__((ext_vector_type(3), aligned(4)));
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int ll = lane % LPC, g = lane / LPC;  // slice and code of this lane in a load
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ec = wide_code<LPC>(lane);  // the code of a round whose trailer and output this lane has
+  const long pstride = prep_stride(s.D);
+  const long lstride = qlimb_stride(s.D);
+  const long nrounds = (n + CPR - 1) / CPR;
+  const long W = (long)gridDim.x * WAVES;
+  const long r0 = (long)blockIdx.x * WAVES + wave;
+  const uint32_t base0 = (uint32_t)(4 * ll), base1 = base0 + 65536u;
+  float cent[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) cent[k] = s.cent[k < s.L ? k : 0];
+
+  for (long qi = 0; qi < m; ++qi) {
+    const float *pq = prep + qi * pstride;
+    const int32_t *L = ql + qi * lstride;
+    __syncthreads();  // the previous query is done with the tables
+    for (int t = tid; t < (D / 2) * 64; t += THREADS) {
+      // t = l + LPC (e + 64 j): lane slice l, entry e, byte j = 4u + kk of
+      // the slice (consecutive threads write consecutive banks).
+      const int l = t % LPC, e = (t / LPC) & 63, j = t / (64 * LPC);
+      const int p = 4 * NW * l + j;  // byte position in the code
+      const int lo = e & 7, hi = e >> 3, kk = j & 3, u = j >> 2;
+      tab[(4 * l + 128 * (u & 1) + 65536 * (u >> 1) + 512 * lo + 8192 * hi + 256 * (kk & 1) +
+           4096 * (kk >> 1)) >> 2] = fmaf(pq[2 * p + 1], cent[hi], pq[2 * p] * cent[lo]);
+    }
+    __syncthreads();
+    const float qns = pq[2 * s.D];
+    const float qn = pq[2 * s.D + 1];
+    const int qe = SYM ? 0 : L[0];
+    const int qsum = SYM ? 0 : L[1];
+    float *orow = out + qi * n;
+    int32_t lw[SYM ? 1 : NW][kQjlDigits];
+    if constexpr (!SYM) {
+#pragma unroll
+      for (int u = 0; u < NW; ++u)
+#pragma unroll
+        for (int j = 0; j < kQjlDigits; ++j) lw[u][j] = L[4 + j * (D / 8) + NW * ll + u];
+    }
+
+    // Ring slot k: this lane's slices of the round's codes and the trailer
+    // of code ec. Every load is unconditional (round clamped to the last one,
+    // code to the last code), so the in-order load counter lets the newer
+    // rounds stay in flight while the oldest is scored.
+    slice_t ring[R][NL];
+    trail_t trl[R];
+    auto load = [&](auto slot, long r) {
+      constexpr int k = decltype(slot)::value;
+      const long rr = r < nrounds ? r : nrounds - 1;
+      __builtin_amdgcn_sched_barrier(0);
+      {
+        long c = rr * CPR + ec;
+        c = c < n ? c : n - 1;
+        trl[k] = __builtin_nontemporal_load(
+            reinterpret_cast<const trail_t *>(codes + c * (long)s.cs + s.packed));
+      }
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        long c = rr * CPR + CPI * i + g;
+        c = c < n ? c : n - 1;
+        ring[k][i] = __builtin_nontemporal_load(
+            reinterpret_cast<const slice_t *>(codes + c * (long)s.cs + 4 * NW * ll));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    };
+
+    long r = r0;
+    asm volatile("" : "+s"(r));  // keeps the first rounds' loads inside the query loop
+    auto step = [&](auto slot) {
+      constexpr int k = decltype(slot)::value;
+      float am[NL];
+      int aq[NL];
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        uint32_t w[NW];
+#pragma unroll
+        for (int u = 0; u < NW; ++u) w[u] = ring[k][i][u];
+        // MSE: two chains of lookups (even and odd bytes), each started by
+        // its first entry.
+        float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+        for (int u = 0; u < NW; ++u) {
+          const uint32_t mw = w[u] & 0xEEEEEEEEu;
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) {
+            const uint32_t addr =
+                __builtin_amdgcn_perm(u >> 1 ? base1 : base0, mw, 0x07060004u | ((uint32_t)kk << 8));
+            const float v = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(tab) + addr +
+                                                             128 * (u & 1) + 256 * (kk & 1) + 4096 * (kk >> 1));
+            if (u == 0 && kk < 2)
+              (kk ? a1 : a0) = v;
+            else if (kk & 1)
+              a1 += v;
+            else
+              a0 += v;
+          }
+        }
+        am[i] = a0 + a1;
+        // QJL: sum_j 16^j sum_u dot8(t_u, digit j), Horner from the top digit.
+        if constexpr (!SYM) {
+          int t[NW];
+#pragma unroll
+          for (int u = 0; u < NW; ++u) t[u] = (int)(w[u] & 0x11111111u);
+          int acc = 0;
+#pragma unroll
+          for (int j = kQjlDigits - 1; j >= 0; --j) {
+            if (j < kQjlDigits - 1) acc <<= 4;
+#pragma unroll
+            for (int u = 0; u < NW; ++u) acc = __builtin_amdgcn_sdot8(t[u], lw[u][j], acc, false);
+          }
+          aq[i] = acc;
+        } else {
+          aq[i] = 0;
+        }
+      }
+      const float xn = trl[k].x, gamma = trl[k].y, sigma = trl[k].z;
+      const long rc = r * CPR + ec;
+      load(slot, r + (long)R * W);
+      const float msum = wide_reduce<LPC>(am, lane);
+      const int T = SYM ? 0 : wide_reduce<LPC>(aq, lane);
+      if ((lane & 7) == 0 && rc < n) {
+        const float ip_mse = msum * sigma;
+        float corr = 0.0f;
+        if constexpr (!SYM) {
+          const float dot = ldexpf((float)(2 * T - qsum), -qe);
+          corr = s.scale * gamma * dot;
+        }
+        const float ip = (ip_mse + corr) * xn * qn;
+        const float d = qns + xn * xn - 2.0f * ip;
+        orow[rc] = d > 0.0f ? d : 0.0f;
+      }
+      r += W;
+    };
+
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, 1>;
+    using S2 = std::integral_constant<int, 2>;
+    static_assert(R == 3, "the unrolled ring below is written for three slots");
+    if (r < nrounds) {
+      // Whole turns of the ring, then the rounds of a last partial turn.
+      const long nr = (nrounds - 1 - r) / W + 1;
+      load(S0{}, r);
+      load(S1{}, r + W);
+      load(S2{}, r + 2 * W);
+      for (long t = nr / 3; t > 0; --t) {
+        step(S0{});
+        step(S1{});
+        step(S2{});
+      }
+      if (nr % 3 >= 1) step(S0{});
+      if (nr % 3 >= 2) step(S1{});
+    }
+  }
+}
+
 // Compute units of the current device (asked per launch: cheap, and correct
 // for every device and thread).
 int cu_count() {
@@ -375,6 +642,21 @@ hipError_t launch_lut_d(const SpaceDev &s, const float *prep, const int32_t *ql,
   return hipGetLastError();
 }
 
+template <bool SYM, int D>
+hipError_t launch_wide_d(const SpaceDev &s, const float *prep, const int32_t *ql, long m,
+                         const uint8_t *codes, long n, float *out, hipStream_t st) {
+  using Cf = WideCfg<D>;
+  const long per_cu = (long)((160 * 1024) / Cf::lds_bytes);
+  long blocks = (long)cu_count() * (per_cu > 0 ? per_cu : 1);
+  const long rounds = (n + kWideCodes - 1) / kWideCodes;
+  const long need = (rounds + Cf::THREADS / 64 - 1) / (Cf::THREADS / 64);
+  if (blocks > need) blocks = need;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL((k_lut_wide<SYM, D>), dim3((unsigned)blocks), dim3(Cf::THREADS), 0, st, s,
+                     prep, ql, m, codes, n, out);
+  return hipGetLastError();
+}
+
 template <bool SYM>
 hipError_t launch_lut(const SpaceDev &s, const float *prep, const int32_t *ql, long m,
                       const uint8_t *codes, long n, float *out, hipStream_t st) {
@@ -383,8 +665,8 @@ hipError_t launch_lut(const SpaceDev &s, const float *prep, const int32_t *ql, l
     case 64: return launch_lut_d<SYM, 64>(s, prep, ql, m, codes, n, out, st);
     case 128: return launch_lut_d<SYM, 128>(s, prep, ql, m, codes, n, out, st);
     case 256: return launch_lut_d<SYM, 256>(s, prep, ql, m, codes, n, out, st);
-    case 512: return launch_lut_d<SYM, 512>(s, prep, ql, m, codes, n, out, st);
-    case 1024: return launch_lut_d<SYM, 1024>(s, prep, ql, m, codes, n, out, st);
+    case 512: return launch_wide_d<SYM, 512>(s, prep, ql, m, codes, n, out, st);
+    case 1024: return launch_wide_d<SYM, 1024>(s, prep, ql, m, codes, n, out, st);
     default: return hipErrorNotSupported;
   }
 }
