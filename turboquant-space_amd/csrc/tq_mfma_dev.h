@@ -122,14 +122,18 @@ __device__ __forceinline__ void decode_nib_word(uint32_t w, const PermTab &t, h8
   }
 }
 
-// Units of 8 consecutive coordinates -> B-operand fragments.
+// Units of 8 consecutive coordinates -> B-operand fragments. l_tab holds
+// (c_hi | c_lo << 16) of centroid k at entry k (REP = false), or 32 copies of
+// it at entries 32 k + c (REP = true) of which lane l reads copy c = l % 32,
+// so that a half-wavefront's lookups fall in 32 distinct banks.
+template <bool REP = false>
 __device__ __forceinline__ void decode8(const uint32_t (&u)[8], const uint32_t *l_tab, h8 &bh,
-                                        h8 &bl, h8 &bt) {
+                                        h8 &bl, h8 &bt, uint32_t copy = 0) {
   uint32_t ph[4], pl[4], pt[4];
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
-    const uint32_t t0 = l_tab[u[2 * p] >> 1];
-    const uint32_t t1 = l_tab[u[2 * p + 1] >> 1];
+    const uint32_t t0 = l_tab[REP ? ((u[2 * p] >> 1) << 5) | copy : u[2 * p] >> 1];
+    const uint32_t t1 = l_tab[REP ? ((u[2 * p + 1] >> 1) << 5) | copy : u[2 * p + 1] >> 1];
     ph[p] = (t0 & 0xFFFFu) | (t1 << 16);
     pl[p] = (t0 >> 16) | (t1 & 0xFFFF0000u);
     pt[p] = 0xBC00BC00u ^ ((u[2 * p] & 1u) << 15) ^ ((u[2 * p + 1] & 1u) << 31);
