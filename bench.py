@@ -693,7 +693,8 @@ def main():
     bytes_1n = n * (cs + 4) + sp.prep_stride * 4
     one_to_n = {"dist_per_s": n / (t1 / 1e3), "kernel_ms": t1,
                 "achieved_GBs": bytes_1n / (t1 / 1e3) / 1e9,
-                "frac_hbm": bytes_1n / (t1 / 1e3) / 1e9 / HBM_PEAK_GBS}
+                "frac_hbm": bytes_1n / (t1 / 1e3) / 1e9 / HBM_PEAK_GBS,
+                "traffic": measured_traffic("k_lut_wide_1ton")}
     log(f"1-to-N: {t1:.3f} ms, {one_to_n['achieved_GBs']:.0f} GB/s")
 
     best = "mfma" if "mfma" in results else "fp32"
@@ -709,7 +710,7 @@ def main():
         bytes_mn = m * n * (cs + 4)
         achieved = bytes_mn / (r["scan_ms"] / 1e3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_lut_nib")}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_lut_wide_mxn")}
         dtype = "f32"
 
     cpu = None
