@@ -289,7 +289,8 @@ def bench_serving(torch, capi, sh, qps_by_mode, seconds, n_per_coll=SERVE_N, see
         lat_s = np.array([d[1] for d in done if reqs[d[0]][0] == "search"])
         lat_a = np.array([d[1] for d in done if reqs[d[0]][0] == "append"])
         nq = sum(reqs[i][2] for i in range(count) if reqs[i][0] == "search")
-        wall = t_end - t_start
+        # Arrivals span `seconds`; the last completions may come later.
+        wall = max(t_end - t_start, seconds)
         return {
             "qps_offered": qps, "qps_realized": count / seconds, "requests": count,
             "searches": int(len(lat_s)), "appends": int(len(lat_a)), "seconds": wall,
