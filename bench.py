@@ -250,40 +250,56 @@ def bench_serving(torch, capi, sh, qps_by_mode, seconds, n_per_coll=SERVE_N, see
         _, x, _ = next(gaussian_mixture(torch, rows, DIM, 8192, 0.3, 5000 + i, centres=centres))
         payload[i] = np.ascontiguousarray(x.cpu().numpy())
 
-    def run_point(srv, ids, qps):
+    def arrivals(qps):
         arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / qps, len(reqs)))
-        count = int(np.searchsorted(arr, seconds))
+        return arr, int(np.searchsorted(arr, seconds))
+
+    def run_point(srv, ids, qps):
+        arr, count = arrivals(qps)
         tickets = [None] * count
         done = []
+        stop = [False]
         lock = threading.Condition()
 
         def waiter():
             for i in range(count):
                 with lock:
-                    while tickets[i] is None:
+                    while tickets[i] is None and not stop[0]:
                         lock.wait()
+                    if tickets[i] is None:
+                        return  # submission stopped early
                 lat = srv.wait(tickets[i])
                 done.append((i, lat, time.perf_counter()))
 
         st0 = srv.stats()
-        th = threading.Thread(target=waiter)
+        th = threading.Thread(target=waiter, daemon=True)
         th.start()
         t_start = time.perf_counter()
-        for i in range(count):
-            delay = t_start + arr[i] - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            kind, c, rows = reqs[i]
-            if kind == "append":
-                t = srv.append(ids[c], payload[i])
-            else:
-                od = np.empty((rows, 10), np.float32)
-                oi = np.empty((rows, 10), np.int64)
-                t = srv.search(ids[c], payload[i], od, oi)
+        err = None
+        try:
+            for i in range(count):
+                delay = t_start + arr[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                kind, c, rows = reqs[i]
+                if kind == "append":
+                    t = srv.append(ids[c], payload[i])
+                else:
+                    od = np.empty((rows, 10), np.float32)
+                    oi = np.empty((rows, 10), np.int64)
+                    t = srv.search(ids[c], payload[i], od, oi)
+                with lock:
+                    tickets[i] = t
+                    lock.notify()
+        except Exception as e:  # a refused submission ends the point (and the run) loudly
+            err = e
+        finally:
             with lock:
-                tickets[i] = t
-                lock.notify()
+                stop[0] = True
+                lock.notify_all()
         th.join()
+        if err is not None:
+            raise err
         t_end = max(d[2] for d in done) if done else time.perf_counter()
         st = srv.stats()
         lat_s = np.array([d[1] for d in done if reqs[d[0]][0] == "search"])
@@ -307,7 +323,14 @@ def bench_serving(torch, capi, sh, qps_by_mode, seconds, n_per_coll=SERVE_N, see
 
     for mode_name, points in qps_by_mode.items():
         mode = capi.TQ_MODE_MFMA if mode_name == "mfma" else capi.TQ_MODE_FP32
-        cap = n_per_coll + 2_000_000
+        # Capacity for every append the sweep can submit (points share the
+        # request pool, and the collections keep what earlier points appended).
+        extra = [0] * SERVE_COLLS
+        for qps in points:
+            for kind, c, rows in reqs[:arrivals(qps)[1]]:
+                if kind == "append":
+                    extra[c] += rows
+        cap = n_per_coll + max(extra) + 10_000
         srv = capi.Server(mode=mode, k=10, hbm_budget=96 << 30, slot_bytes=2 << 30,
                           max_inflight=2, max_batch_queries=4096)
         ids = []

@@ -84,6 +84,7 @@ void tune_store(int D, int bits, long m, const tq_search_tuning &t) {
 }
 
 constexpr int kDefaultItemsPerWg = 4;
+constexpr long kMinItemTiles = 8;  // fused form: code tiles per work item, at least
 constexpr size_t kFp32BlockBytes = 512ull << 20;  // distance rows per block (FP32 plans)
 // MFMA plans: by default a batch runs blocked: k_mfma writes the distances
 // of all its queries against a chunk of codes into a block of at most
@@ -279,7 +280,7 @@ int layout(tq_search_plan_s *p) {
         const long ntiles = (b.n + ntile - 1) / ntile;
         total += (double)qtiles * ntiles * b.space->info.padded_dim * kind;
       }
-      const long wg = (long)p->cus * (kind == 1 ? 2 : 1);
+      const long wg = (long)p->cus * 2;  // two workgroups per CU for both query tiles
       const double target = total / (double)(wg * ipw);
       for (size_t bi = 0; bi < p->batches.size(); ++bi) {
         const PlanBatch &b = p->batches[bi];
@@ -288,6 +289,11 @@ int layout(tq_search_plan_s *p) {
         const long ntiles = (b.n + ntile - 1) / ntile;
         const double tile_cost = (double)b.space->info.padded_dim * kind;
         long tpi = (long)std::llround(target / tile_cost);
+        // At least kMinItemTiles code tiles per item: every item writes 256
+        // candidate lists and the merge reads them per query, so one-tile
+        // items (small batches over many workgroups) cost more than they
+        // balance.
+        tpi = std::max(tpi, tq::kMinItemTiles);
         tpi = std::max(1L, std::min(tpi, std::max(ntiles, 1L)));
         for (long qt = 0; qt < qtiles; ++qt) {
           tq::SearchGroup g{(int)bi, (int)qt, 0, 0, kind};
@@ -308,14 +314,23 @@ int layout(tq_search_plan_s *p) {
     for (const Pending &pi : pend) p->h_items.push_back(pi.it);
     std::vector<int> ord(pend.size());
     for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
+    // Queue order: per kind, the costliest items first; among items of equal
+    // cost, by code range and then query tile, so the workgroups that run
+    // side by side score the same codes for different query tiles and share
+    // the code bytes in L2 (items of one query tile stay contiguous in the
+    // item list, which the groups index; only the queue visits them apart).
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
-      if (pend[a].kind != pend[b].kind) return pend[a].kind < pend[b].kind;
-      return pend[a].cost > pend[b].cost;
+      const Pending &x = pend[a], &y = pend[b];
+      if (x.kind != y.kind) return x.kind < y.kind;
+      if (x.cost != y.cost) return x.cost > y.cost;
+      if (x.it.job != y.it.job) return x.it.job < y.it.job;
+      if (x.it.t0 != y.it.t0) return x.it.t0 < y.it.t0;
+      return x.it.qtile < y.it.qtile;
     });
     p->h_order = ord;
     for (const Pending &pi : pend) p->nitems_qr[pi.kind - 1]++;
     for (int kind = 1; kind <= 2; ++kind) {
-      const long wg = (long)p->cus * (kind == 1 ? 2 : 1);
+      const long wg = (long)p->cus * 2;  // two workgroups per CU for both query tiles
       p->grid_qr[kind - 1] = (int)std::min<long>(wg, p->nitems_qr[kind - 1]);
     }
   }

@@ -2,20 +2,20 @@
 // as one persistent launch over a queue of (job, query tile, code range)
 // items. Jobs may come from different collections, dims and bit widths.
 //
-// Per item a workgroup walks its code tiles like k_mfma (tq_mfma.hip): 256
-// codes per tile, 32-coordinate chunks staged in LDS, fp16 hi+lo MFMA passes
-// and the reference's fp32 epilogue (space_turbo_quant.h:682-685), over a
-// query tile of 32 (QR = 1, two workgroups per CU) or 64 queries (QR = 2, one
-// workgroup per CU, room for 2 x 2 accumulator blocks). Instead of writing the
-// distances, each wavefront keeps the running best K (distance, code id)
-// entries of the codes it has seen in per-lane lists in registers: with
-// QR = 2 lane q holds query q's list; with QR = 1 lanes q and q + 32 hold
-// query q's lists for the two halves of the wavefront's 64 codes. A tile's
-// distances reach the lists through an LDS transpose; a wavefront-wide test
-// against the lists' K-th entries skips candidates no list takes. Every
-// (query, code) distance is the same arithmetic whatever else is in the
-// launch, and ties break by id, so a query's result does not depend on the
-// batch it was scored in.
+// Per item a workgroup walks its code tiles with k_mfma's tile contraction
+// (mfma_tile, tq_mfma_tile.h: 256 codes per tile, 32-coordinate chunks
+// through a register ring and LDS, the fp16 hi+lo MFMA passes) and the
+// reference's fp32 epilogue (space_turbo_quant.h:682-685), over a query tile
+// of 32 (QR = 1) or 64 queries (QR = 2), two workgroups per CU. Instead of
+// writing the distances, each wavefront keeps the running best K (distance,
+// code id) entries of the codes it has seen in 64 per-lane lists in LDS:
+// with QR = 2 lane q holds query q's list; with QR = 1 lanes q and q + 32
+// hold query q's lists for the two halves of the wavefront's 64 codes. A
+// tile's distances reach the lists through an LDS transpose; a lane reads
+// its list into registers only when one of the wavefront's distances beats
+// a list's K-th entry. Every (query, code) distance is the same arithmetic
+// whatever else is in the launch (and the same as k_mfma's), and ties break
+// by id, so a query's result does not depend on the batch it was scored in.
 //
 // k_topk_merge then takes, per query, the best k of the lists of the items
 // that covered it. This is the fused form of a search; k_topk_rows and
@@ -26,7 +26,7 @@
 // run_benchmark.py:251-255); ties there are unordered, here they are ordered
 // by id.
 #include "tq_mfma.h"
-#include "tq_mfma_dev.h"
+#include "tq_mfma_tile.h"
 #include "tq_search.h"
 
 namespace tq {
@@ -35,9 +35,6 @@ namespace {
 using namespace mfma;
 
 constexpr int kSN = kSearchNTile;  // codes per tile (64 per wavefront)
-constexpr int kSKC = 32;           // coordinates per chunk
-constexpr int kSPlanes = 4;        // q_hi, q_lo, s_hi, s_lo (byte spaces read three)
-constexpr int kSCRow = 36;         // padded code row in bytes (8-bit chunks need 32)
 constexpr int kTRow = 65;          // padded transpose row, in floats
 
 __device__ __forceinline__ bool lex_less(float a, uint32_t ia, float b, uint32_t ib) {
@@ -111,272 +108,192 @@ __global__ void k_reset_counter(int *counter) { *counter = 0; }
 template <int QR>
 struct SearchShape {
   static constexpr int QT = 32 * QR;  // queries per item
-  static constexpr int WPE = 3 - QR;  // waves per SIMD the registers allow
-  static constexpr int NBMAX = kSPlanes * QR * 2;  // 1 KB query-plane blocks per chunk
-  static constexpr int ABYTES = NBMAX * 1024;
-  static constexpr int STAGE = 2 * ABYTES + 2 * kSN * kSCRow;
-  static constexpr int TRANS = 4 * 32 * kTRow * 4;
-  static constexpr int LDS = STAGE > TRANS ? STAGE : TRANS;
-  static constexpr int ALOADS = NBMAX * 64 / 256;  // 16-byte query-plane loads per thread
+  using ShN = MfmaShape<true, QT, true>;
+  using ShB = MfmaShape<false, QT, true>;
+  static constexpr int STAGE = ShN::STAGE > ShB::STAGE ? ShN::STAGE : ShB::STAGE;
+  static constexpr int TRANS = 4 * 32 * kTRow * 4;  // one 32-row distance block per wavefront
+  static constexpr int LDS = 2 * STAGE > TRANS ? 2 * STAGE : TRANS;
 };
 
-template <int QR, int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SearchShape<QR>::WPE,
-                                                                     SearchShape<QR>::WPE))) void
-k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ items,
-         const int *__restrict__ order, int nitems, int *__restrict__ counter,
-         float *__restrict__ cand_d, uint32_t *__restrict__ cand_i) {
-  using Sh = SearchShape<QR>;
-  constexpr int QT = Sh::QT;
-  constexpr int ALOADS = Sh::ALOADS;
-  // The chunk staging buffers and the per-tile transpose share one LDS area.
-  __shared__ __attribute__((aligned(16))) uint8_t lds[Sh::LDS];
-  __shared__ uint32_t l_tab[256];
-  __shared__ float l_qn[QT][2];
-  __shared__ int s_item;
-  auto a_lds = reinterpret_cast<_Float16(*)[Sh::ABYTES / 2]>(lds);
-  auto c_lds = reinterpret_cast<uint32_t(*)[kSN * kSCRow / 4]>(lds + 2 * Sh::ABYTES);
-  auto t_lds = reinterpret_cast<float(*)[32][kTRow]>(lds);
-
+// One code tile of an item: the tile's contraction (mfma_tile, the same as
+// k_mfma's), the reference epilogue, and the distances into the
+// wavefront's lists, one 32-query block at a time through a transpose.
+// QR = 2: lanes 32 r .. 32 r + 31 own the queries of block r and read all 64
+// codes of the wavefront; QR = 1: lanes q and q + 32 take codes 0-31 and
+// 32-63 of query q. A lane's list (lane l of wavefront w: l_ld / l_li
+// [w][0..K-1][l], ascending, empty entries (inf, ~0) last) is read into
+// registers only when one of the wavefront's distances beats a list's kk-th
+// entry (kk = the k asked for), and written back after.
+template <bool NIB, int QR, int K>
+__device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, long j0, int kk,
+                                            uint8_t *lds, const uint32_t *l_tab,
+                                            const PermTab &ptab, const float (*l_qn)[2],
+                                            float (*l_ld)[K][64], uint32_t (*l_li)[K][64]) {
+  constexpr int QT = 32 * QR;
+  using Sh = MfmaShape<NIB, QT, true>;
+  const SpaceDev &s = job.s;
+  const long m = job.m, n = job.n;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int lr = lane & 31, kg = lane >> 5;
+  auto stage = reinterpret_cast<uint8_t(*)[Sh::STAGE]>(lds);
+  auto t_lds = reinterpret_cast<float(*)[32][kTRow]>(lds);
+
+  f16v acc_m[QR][2], acc_q[QR][2];
+  __syncthreads();  // the previous tile's transpose reads are done with the stage area
+  mfma_tile<NIB, QT, true, false>(s, job.planes, m, job.codes, n, q0, j0, stage, l_tab, ptab,
+                                  acc_m, acc_q);
+
+  const long jbase = j0 + wave * 64;
+  float fm[2], fq[2], xnv[2], xns[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const long j = jbase + 32 * c + lr;
+    const float *meta =
+        reinterpret_cast<const float *>(job.codes + (j < n ? j : n - 1) * (long)s.cs + s.packed);
+    const float xn = meta[0], gamma = meta[1], sigma = meta[2];
+    fm[c] = sigma * s.mse_unscale;
+    fq[c] = s.scale * gamma * kQjlUnscale;
+    xnv[c] = xn;
+    xns[c] = xn * xn;
+  }
+  const int h = QR == 1 ? (lane >> 5) : 0;
+  const long cbase = jbase + 32 * h;
+#pragma unroll
+  for (int r = 0; r < QR; ++r) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int row = 8 * (v >> 2) + 4 * kg + (v & 3);
+        const int qrow = 32 * r + row;
+        t_lds[wave][row][32 * c + lr] = mfma_distance<true>(acc_m[r][c][v], acc_q[r][c][v], fm[c],
+                                                            fq[c], xnv[c], xns[c], l_qn[qrow][0],
+                                                            l_qn[qrow][1]);
+      }
+    __syncthreads();
+    const bool mine = QR == 1 || (lane >> 5) == r;
+    const float *rowp = &t_lds[wave][lane & 31][32 * h];
+    // Candidates: the lane's codes below its list's kk-th entry (only a
+    // query's best kk can reach its result), as a bit mask over the block.
+    const float thd = l_ld[wave][kk - 1][lane];
+    const uint32_t thi = l_li[wave][kk - 1][lane];
+    constexpr int NK = 64 / (3 - QR);  // codes a lane scores in this block
+    uint64_t cm = 0;
+    if (mine && jbase < n) {
+      for (int k = 0; k < NK; ++k)
+        if (cbase + k < n && lex_less(rowp[k], (uint32_t)(cbase + k), thd, thi)) cm |= 1ull << k;
+    }
+    if (__any(cm != 0)) {
+      float td[K];
+      uint32_t ti[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        td[k] = l_ld[wave][k][lane];
+        ti[k] = l_li[wave][k][lane];
+      }
+      // One candidate per lane per round (lowest code first, so ids enter in
+      // ascending order as before); lanes without one insert the empty entry
+      // (a no-op). Rounds = the most candidates any lane has.
+      while (__any(cm != 0)) {
+        float d = __builtin_inff();
+        uint32_t id = 0xFFFFFFFFu;
+        if (cm != 0) {
+          const int k = __builtin_ctzll(cm);
+          cm &= cm - 1;
+          d = rowp[k];
+          id = (uint32_t)(cbase + k);
+        }
+        topk_insert<K>(td, ti, d, id);
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        l_ld[wave][k][lane] = td[k];
+        l_li[wave][k][lane] = ti[k];
+      }
+    }
+    __syncthreads();  // block r's transpose reads are done before block r + 1 is written
+  }
+}
+
+// Persistent fused search: workgroups pull (job, query tile, code-tile
+// range) items from a queue; per item each wavefront keeps 64 lists of K
+// (distance, id) entries in LDS (lane l's list: its query's best entries
+// among the codes it scored) and writes them to the candidate buffer at the
+// end: entries ((item * 4 + wavefront) * 64 + lane) * K. Two workgroups per
+// CU for both query tiles (the lists live in LDS, the tile's registers are
+// k_mfma's).
+template <int QR, int K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ items,
+         const int *__restrict__ order, int nitems, int *__restrict__ counter, int kk,
+         float *__restrict__ cand_d, uint32_t *__restrict__ cand_i) {
+  using Sh = SearchShape<QR>;
+  constexpr int QT = Sh::QT;
+  // The stage buffers of the tile's contraction and the distance transpose
+  // share one LDS area.
+  __shared__ __attribute__((aligned(16))) uint8_t lds[Sh::LDS];
+  __shared__ uint32_t l_tab[128];
+  __shared__ float l_qn[QT][2];
+  __shared__ float l_ld[4][K][64];
+  __shared__ uint32_t l_li[4][K][64];
+  __shared__ int s_item;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
 
   for (;;) {
     __syncthreads();  // the previous item is done with every shared buffer
     if (tid == 0) s_item = atomicAdd(counter, 1);
     __syncthreads();
-    const int pos = s_item;
+    // The same value in every lane: read into scalar registers, so the
+    // item and job loads below are scalar and stay in SGPRs.
+    const int pos = __builtin_amdgcn_readfirstlane(s_item);
     if (pos >= nitems) break;  // every wavefront reads the same value and leaves together
-    const int it = order[pos];  // queue position -> item
+    const int it = __builtin_amdgcn_readfirstlane(order[pos]);  // queue position -> item
     const SearchItem item = items[it];
     const SearchJobDev &job = jobs[item.job];
-    const SpaceDev s = job.s;
+    const SpaceDev &s = job.s;
     const bool nib = s.nib != 0;
-    const long m = job.m, n = job.n;
+    const long m = job.m;
     const long q0 = (long)item.qtile * QT;
-    const int nchunks = s.D / kSKC;
-    const int nks = s.D / 16;
-    const int cparts = nib ? 1 : 2;  // 16-byte loads per code and chunk
-    const int npl = split_planes(nib, true);
-    const int nb = npl * QR * 2;    // query-plane blocks per chunk
 
-    for (int i = tid; i < 256; i += blockDim.x) {
-      _Float16 ch, cl;
-      cent_h(s, i, ch, cl);
-      l_tab[i] = h_bits(ch) | (h_bits(cl) << 16);
-    }
+    if (!nib)
+      for (int i = tid; i < 128; i += blockDim.x) {
+        _Float16 ch, cl;
+        cent_h(s, i, ch, cl);
+        l_tab[i] = h_bits(ch) | (h_bits(cl) << 16);
+      }
     if (tid < QT) {
       const long q = q0 + tid;
       const float *pq = job.prep + (q < m ? q : 0L) * prep_stride(s.D);
       l_qn[tid][0] = pq[2 * s.D];
       l_qn[tid][1] = pq[2 * s.D + 1];
     }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      l_ld[wave][k][lane] = __builtin_inff();
+      l_li[wave][k][lane] = 0xFFFFFFFFu;
+    }
     PermTab ptab{};
     if (nib) ptab = perm_tab(s);
-
-    // Query-plane loads: the chunk's nb blocks of 1 KB (tq_mfma_dev.h),
-    // block i = (row block r, k step kb, plane p), i = (r * 2 + kb) * npl + p,
-    // 16 bytes per thread and load. Query blocks past m read the last block.
-    const long nrb = (m + 31) / 32;
-    const _Float16 *asrc[ALOADS];
-    bool ain[ALOADS];
-    int adst[ALOADS];
-#pragma unroll
-    for (int i = 0; i < ALOADS; ++i) {
-      const int idx0 = i * 256 + tid;
-      ain[i] = idx0 < nb * 64;
-      const int idx = ain[i] ? idx0 : 0;
-      const int blk = idx >> 6, l = idx & 63;
-      const int r = blk / (2 * npl), kb = (blk / npl) & 1, p = blk % npl;
-      long rb = q0 / 32 + r;
-      if (rb >= nrb) rb = nrb - 1;
-      asrc[i] = job.planes + ((rb * nks + kb) * npl + p) * (long)kBlockHalves + 8 * l;
-      adst[i] = idx * 8;
-    }
-
-    float td[K];
-    uint32_t ti[K];
-    topk_clear<K>(td, ti);
-
+    // (mfma_tile starts with the first chunk's loads; search_tile's barrier
+    // orders the tables above before its first use of them.)
     for (int tile = item.t0; tile < item.t1; ++tile) {
       const long j0 = (long)tile * kSN;
-      // Code loads: 256 codes x cparts x 16 B (1 or 2 per thread).
-      const uint8_t *csrc[2];
-      bool cok[2];
-      int cdst[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int idx = i * 256 + tid;
-        const int c = nib ? idx : idx >> 1;
-        const int part = nib ? 0 : idx & 1;
-        const long j = j0 + c;
-        cok[i] = (i < cparts) && j < n;
-        csrc[i] = job.codes + (cok[i] ? j : 0L) * (long)s.cs + 16 * part;
-        cdst[i] = (c * kSCRow + 16 * part) / 4;
-      }
-      uint4 ra[ALOADS], rc[2];
-      auto load_chunk = [&](int ch) {
-        const int k0 = ch * kSKC;
-#pragma unroll
-        for (int i = 0; i < ALOADS; ++i)
-          ra[i] = ain[i] ? *reinterpret_cast<const uint4 *>(asrc[i] + (long)ch * 2 * npl * kBlockHalves)
-                         : make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          if (cok[i]) {
-            const uint32_t *src = reinterpret_cast<const uint32_t *>(csrc[i] + (nib ? k0 / 2 : k0));
-            rc[i] = make_uint4(src[0], src[1], src[2], src[3]);
-          } else {
-            rc[i] = make_uint4(0u, 0u, 0u, 0u);
-          }
-        }
-      };
-      auto store_chunk = [&](int b) {
-        _Float16 *abuf = &a_lds[b][0];
-#pragma unroll
-        for (int i = 0; i < ALOADS; ++i)
-          if (ain[i]) *reinterpret_cast<uint4 *>(abuf + adst[i]) = ra[i];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          if (i < cparts) {
-            uint32_t *dst = &c_lds[b][cdst[i]];
-            dst[0] = rc[i].x;
-            dst[1] = rc[i].y;
-            dst[2] = rc[i].z;
-            dst[3] = rc[i].w;
-          }
-        }
-      };
-
-      // The tile's contraction and its distances into the lists: the passes
-      // and their order are k_mfma's (the s_lo pass for nibble spaces only),
-      // compiled per kind (a branch around MFMAs inside the unrolled loop
-      // would split its schedule).
-      auto scan_tile = [&](auto nib_c) __attribute__((always_inline)) {
-        constexpr bool NIB = decltype(nib_c)::value;
-        constexpr int NPL = split_planes(NIB, true);
-        f16v acc_m[QR][2], acc_q[QR][2];
-#pragma unroll
-        for (int r = 0; r < QR; ++r)
-#pragma unroll
-          for (int c = 0; c < 2; ++c)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-              acc_m[r][c][v] = 0.0f;
-              acc_q[r][c][v] = 0.0f;
-            }
-
-        __syncthreads();  // the previous tile's transpose reads are done
-        load_chunk(0);
-        store_chunk(0);
-        __syncthreads();
-        for (int ch = 0; ch < nchunks; ++ch) {
-          const int b = ch & 1;
-          if (ch + 1 < nchunks) load_chunk(ch + 1);
-#pragma unroll
-          for (int kb = 0; kb < kSKC / 16; ++kb) {
-            h8 a[QR][NPL];
-#pragma unroll
-            for (int r = 0; r < QR; ++r)
-#pragma unroll
-              for (int p = 0; p < NPL; ++p)
-                a[r][p] = *reinterpret_cast<const h8 *>(
-                    &a_lds[b][((r * 2 + kb) * NPL + p) * kBlockHalves + 8 * lane]);
-            h8 bh[2], bl[2], bt[2];
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-              const int code = wave * 64 + 32 * c + lr;
-              if constexpr (NIB) {
-                const uint32_t w = c_lds[b][(code * kSCRow + 8 * kg + 4 * kb) / 4];
-                decode_nib_word(w, ptab, bh[c], bl[c], bt[c]);
-              } else {
-                const uint32_t w0 = c_lds[b][(code * kSCRow + 16 * kg + 8 * kb) / 4];
-                const uint32_t w1 = c_lds[b][(code * kSCRow + 16 * kg + 8 * kb) / 4 + 1];
-                uint32_t u[8];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                  u[v] = (w0 >> (8 * v)) & 0xFFu;
-                  u[4 + v] = (w1 >> (8 * v)) & 0xFFu;
-                }
-                decode8(u, l_tab, bh[c], bl[c], bt[c]);
-              }
-            }
-#define TQ_PASS(ACC, A, B)                                                               \
-    _Pragma("unroll") for (int r = 0; r < QR; ++r)                                         \
-        _Pragma("unroll") for (int c = 0; c < 2; ++c) ACC[r][c] =                          \
-            __builtin_amdgcn_mfma_f32_32x32x16_f16(a[r][A], B[c], ACC[r][c], 0, 0, 0);
-            TQ_PASS(acc_m, 0, bh)
-            TQ_PASS(acc_q, 2, bt)
-            TQ_PASS(acc_m, 1, bh)
-            if constexpr (NPL > 3) { TQ_PASS(acc_q, 3, bt) }
-            TQ_PASS(acc_m, 0, bl)
-#undef TQ_PASS
-          }
-          if (ch + 1 < nchunks) store_chunk(b ^ 1);
-          __syncthreads();
-        }
-
-        // Distances (the reference epilogue, as in k_mfma) into the lists, one
-        // 32-query block at a time through the transpose area.
-        const long jbase = j0 + wave * 64;
-        float fm[2], fq[2], xnv[2], xns[2];
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          const long j = jbase + 32 * c + lr;
-          const float *meta =
-              reinterpret_cast<const float *>(job.codes + (j < n ? j : 0L) * (long)s.cs + s.packed);
-          const float xn = meta[0], gamma = meta[1], sigma = meta[2];
-          fm[c] = sigma * s.mse_unscale;
-          fq[c] = s.scale * gamma * kQjlUnscale;
-          xnv[c] = xn;
-          xns[c] = xn * xn;
-        }
-#pragma unroll
-        for (int r = 0; r < QR; ++r) {
-#pragma unroll
-          for (int c = 0; c < 2; ++c)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-              const int row = 8 * (v >> 2) + 4 * kg + (v & 3);
-              const int qrow = 32 * r + row;
-              const float ip = (acc_m[r][c][v] * fm[c] + acc_q[r][c][v] * fq[c]) * xnv[c] * l_qn[qrow][1];
-              const float d = l_qn[qrow][0] + xns[c] - 2.0f * ip;
-              t_lds[wave][row][32 * c + lr] = d > 0.0f ? d : 0.0f;
-            }
-          __syncthreads();
-          // QR = 2: lanes 32r .. 32r+31 own this block's queries and read all
-          // 64 codes. QR = 1: lane q and q + 32 read codes 0-31 and 32-63 of
-          // query q. Codes past n and lanes that own nothing here offer the
-          // empty entry (inf, ~0), which no list takes.
-          const int h = QR == 1 ? (lane >> 5) : 0;
-          const bool mine = QR == 1 || (lane >> 5) == r;
-          const float *rowp = &t_lds[wave][lane & 31][32 * h];
-          const long cbase = jbase + 32 * h;
-          if (jbase < n) {
-            for (int k = 0; k < 64 / (3 - QR); ++k) {
-              const bool ok = mine && cbase + k < n;
-              const float d = ok ? rowp[k] : __builtin_inff();
-              const uint32_t id = ok ? (uint32_t)(cbase + k) : 0xFFFFFFFFu;
-              if (__any(lex_less(d, id, td[K - 1], ti[K - 1]))) topk_insert<K>(td, ti, d, id);
-            }
-          }
-          __syncthreads();
-        }
-      };
       if (nib)
-        scan_tile(std::true_type{});
+        search_tile<true, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_ld, l_li);
       else
-        scan_tile(std::false_type{});
+        search_tile<false, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_ld, l_li);
     }
 
     // This wavefront's lists for the item.
     const long base = (((long)it * 4 + wave) * 64 + lane) * K;
-#pragma unroll
     for (int k = 0; k < K; ++k) {
-      cand_d[base + k] = td[k];
-      cand_i[base + k] = ti[k];
+      cand_d[base + k] = l_ld[wave][k][lane];
+      cand_i[base + k] = l_li[wave][k][lane];
     }
   }
 }
@@ -821,10 +738,11 @@ hipError_t launch_search(const SearchJobDev *d_jobs, const SearchItem *d_items,
     if (e != hipSuccess) return e;
     if (qr == 1)
       hipLaunchKernelGGL((k_search<1, kSearchMaxK>), dim3((unsigned)grid_qr[0]), dim3(256), 0, st,
-                         d_jobs, d_items, d_order + off, cnt, d_counter, d_cand_d, d_cand_i);
+                         d_jobs, d_items, d_order + off, cnt, d_counter, k, d_cand_d, d_cand_i);
     else
       hipLaunchKernelGGL((k_search<2, kSearchMaxK>), dim3((unsigned)grid_qr[1]), dim3(256), 0, st,
-                         d_jobs, d_items, d_order + off, cnt, d_counter + 1, d_cand_d, d_cand_i);
+                         d_jobs, d_items, d_order + off, cnt, d_counter + 1, k, d_cand_d,
+                         d_cand_i);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     off += cnt;
