@@ -8,6 +8,7 @@ import os
 import subprocess
 import sys
 import sysconfig
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -38,11 +39,41 @@ def _run(cmd):
         raise RuntimeError("build step failed: " + " ".join(cmd))
 
 
+def _jobs():
+    try:
+        n = int(os.environ.get("MAX_JOBS", "0"))
+    except ValueError:
+        n = 0
+    return max(1, min(n or os.cpu_count() or 1, 16, len(HIP_SOURCES)))
+
+
 def build_hip():
+    """Each source compiles to its own object in parallel (every kernel is
+    launched from the file that defines it, so no relocatable device code is
+    needed); one link makes the shared library."""
     out = os.path.join(PKG, "libtq_hip.so")
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    _run([HIPCC] + HIP_FLAGS + ["-o", out] + srcs)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"] + ["-c"]
+    objs, cmds = [], []
+    for s in HIP_SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
+        objs.append(obj)
+        cmds.append([HIPCC] + compile_flags + ["-o", obj, os.path.join(CSRC, s)])
+    with ThreadPoolExecutor(_jobs()) as pool:
+        errs = [e for e in pool.map(_try_run, cmds) if e]
+    if errs:
+        raise RuntimeError("build step failed: " + "; ".join(errs))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out] + objs)
     return out
+
+
+def _try_run(cmd):
+    try:
+        _run(cmd)
+        return None
+    except RuntimeError as e:
+        return str(e)
 
 
 def build_module():

@@ -128,6 +128,47 @@ struct DevBuf {
   }
 };
 
+
+// Centroid scale of the fp16 MFMA operand. Byte codes and 2-3-bit nibble
+// codes keep 2^8 and carry each centroid as c_hi + c_lo (two fp16 terms).
+// The coarse form of 4-bit codes (tq::mfma_coarse) carries c_hi alone, so its
+// scale alpha in [256, 512) (multiples of 2^-12, exact in fp32) is chosen so
+// that fp16(fl32(c * alpha)) / alpha is within 2^-15 of each of the four
+// centroid magnitudes (1.85e-5 for the 3-bit Lloyd-Max codebook). The search
+// mirrors the device arithmetic: fp32 product, fp16 round to nearest even,
+// fp32 reciprocal scale 1 / (2^10 alpha).
+double fp16_rne(double v) {  // v > 0 in fp16's normal range
+  int e = 0;
+  std::frexp(v, &e);  // v = f * 2^e, f in [0.5, 1)
+  const double ulp = std::ldexp(1.0, e - 11);
+  return std::nearbyint(v / ulp) * ulp;
+}
+
+float coarse_centroid_scale(const std::vector<float> &cent) {
+  static std::mutex mu;
+  static float cached = 0.0f;
+  std::lock_guard<std::mutex> lk(mu);
+  if (cached != 0.0f) return cached;  // one 3-bit codebook for every 4-bit space
+  const size_t L = cent.size();
+  double best = 1.0, best_a = 256.0;
+  for (int k = 0; k < (1 << 20); ++k) {
+    const double a = 256.0 + std::ldexp((double)k, -12);
+    const double u = (double)(float)(1.0 / (1024.0 * a));
+    double worst = 0.0;
+    for (size_t l = L / 2; l < L && worst < best; ++l) {
+      const double c = std::fabs((double)cent[l]);
+      const double v = (double)(float)(c * a);
+      worst = std::max(worst, std::fabs(fp16_rne(v) * u * 1024.0 - c) / c);
+    }
+    if (worst < best) {
+      best = worst;
+      best_a = a;
+    }
+  }
+  cached = best <= std::ldexp(1.0, -15) ? (float)best_a : 256.0f;
+  return cached;
+}
+
 }  // namespace
 
 extern "C" {
@@ -207,9 +248,10 @@ int tq_space_create(uint64_t dim, int bits, uint64_t rot_seed, uint64_t qjl_seed
   d.scale = s->info.scale;
   d.inv_sqrt_D = 1.0f / std::sqrt(static_cast<float>(D));
   // Centroids times 2^8 stay within fp16's normal range together with their
-  // fp16 remainders (|c| < 4 at every bit width).
-  d.cscale = 256.0f;
-  d.mse_unscale = 1.0f / 262144.0f;
+  // fp16 remainders (|c| < 4 at every bit width); the coarse form of 4-bit
+  // spaces takes its single-term scale (coarse_centroid_scale).
+  d.cscale = tq::mfma_coarse(d) ? coarse_centroid_scale(s->cent) : 256.0f;
+  d.mse_unscale = (float)(1.0 / (1024.0 * (double)d.cscale));
   d.cent = s->d_tables;
   d.bnd = s->d_tables + L;
   d.rot = s->d_tables + 2 * L;

@@ -23,8 +23,10 @@ struct SpaceDev {
   float scale;      // sqrt(pi/2)/sqrt(D), as the reference computes it in fp32
   float inv_sqrt_D; // 1.0f / sqrtf((float)D), correctly rounded
   // MFMA forms: the centroids enter the fp16 B operand as fp16(c * cscale)
-  // plus the fp16 remainder; mse_unscale = 1 / (2^10 * cscale) (exact)
-  // undoes the query and centroid scales in the epilogue.
+  // plus the fp16 remainder (the coarse form of 4-bit spaces: the first term
+  // alone, cscale chosen to make it exact to 2^-15); mse_unscale =
+  // fl32(1 / (2^10 * cscale)) undoes the query and centroid scales in the
+  // epilogue.
   float cscale;
   float mse_unscale;
   const float *cent;  // [L]   Lloyd-Max centroids (ascending)
@@ -32,6 +34,11 @@ struct SpaceDev {
   const float *rot;   // [D]   +-1 rotation signs (rot_seed)
   const float *qjl;   // [D]   +-1 QJL signs (qjl_seed)
 };
+
+// MFMA forms: 4-bit spaces (3 MSE bits + the QJL bit) run the coarse pass
+// set (q_hi*c_hi, q_lo*c_hi, s_hi*(+-1)) with the near pairs recomputed in
+// fp32 (refine_pair, tq_mfma_tile.h); other bit widths run every lo pass.
+__host__ __device__ inline bool mfma_coarse(const SpaceDev &s) { return s.nib && s.bits == 4; }
 
 // fp32 trailer field at any byte alignment (4-bit codes at D = 4 are 14 bytes
 // long, so their trailers are only 2-byte aligned).

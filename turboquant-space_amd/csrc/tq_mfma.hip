@@ -19,6 +19,13 @@
 // fp16's normal range; the epilogue removes the scales. With every lo pass
 // in, the error against the fp64 restatement of the estimator is a few 1e-6
 // relative to the distance on each query's nearest codes.
+// 4-bit spaces run a coarse pass set instead (mfma_coarse): q_hi*c_hi,
+// q_lo*c_hi and s_hi*(+-1), three passes instead of five. The dropped terms
+// move a distance by up to a few 1e-5 of (|q|^2 + |x|^2); so every pair whose
+// coarse distance is below half of that scale (a query's near codes, the
+// distances a top-k decides on) is recomputed in fp32 in the epilogue
+// (refine_block, tq_mfma_tile.h), and the pairs above it stay within a few
+// 1e-5 relative. Both parts are functions of the (query, code) pair alone.
 //
 // Tiling: a workgroup (4 wavefronts) owns 64 (or 32) queries x 256 codes;
 // each wavefront owns the queries x 64 codes as 2 x 2 (or 1 x 2) blocks of
@@ -40,10 +47,11 @@ namespace {
 using namespace mfma;
 
 // One thread per (query block, k step, lane): the lane's 8 elements of every
-// plane (layout in tq_mfma_dev.h). Rows past m are zero.
+// plane (layout in tq_mfma_dev.h), and with `corr` the coarse form's
+// correction row group. Rows past m are zero.
 __global__ __launch_bounds__(256) void k_split_queries(SpaceDev s, const float *__restrict__ prep,
                                                        long m, _Float16 *__restrict__ planes,
-                                                       int npl) {
+                                                       int npl, _Float16 *__restrict__ corr) {
   const long stride = prep_stride(s.D);
   const int nks = s.D / 16;
   const long total = (m + 31) / 32 * (long)nks * 64;
@@ -83,6 +91,11 @@ __global__ __launch_bounds__(256) void k_split_queries(SpaceDev s, const float *
     dst[64] = lq;
     if (npl > 2) dst[128] = hs;
     if (npl > 3) dst[192] = ls;
+    if (corr) {
+      h8 *cd = reinterpret_cast<h8 *>(corr + (q * (s.D / 8) + base / 8) * kCorrHalves);
+      cd[0] = hq;
+      cd[1] = ls;
+    }
   }
 }
 
@@ -93,11 +106,11 @@ __global__ __launch_bounds__(256) void k_split_queries(SpaceDev s, const float *
 // the same bits. QJL = false drops the s_q planes and their passes (queries
 // with s_q = 0: the symmetric MSE estimator, whose "queries" are decoded
 // codes); the QJL term is then exactly zero.
-template <bool NIB, int QT, bool QJL>
+template <bool NIB, int QT, bool QJL, bool CO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma(
     SpaceDev s, const _Float16 *__restrict__ planes, const float *__restrict__ prep, long m,
     const uint8_t *__restrict__ codes, long n, float *__restrict__ out) {
-  using Sh = MfmaShape<NIB, QT, QJL>;
+  using Sh = MfmaShape<NIB, QT, QJL, CO>;
   constexpr int RB = Sh::RB;
   static_assert(QT == 32 || QT == 64, "query tile");
   __shared__ __attribute__((aligned(16))) uint8_t stage[2][Sh::STAGE];
@@ -105,6 +118,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   // entry k at 32 k + c), lane l reading copy l % 32 (decode8<true>).
   __shared__ uint32_t l_tab[NIB ? 1 : 128 * 32];
   __shared__ float l_qn[QT][2];  // q_norm_sq, q_norm of the tile's queries
+  __shared__ float l_clo[8];     // coarse form: the centroid remainders, for refine_block
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -134,6 +148,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     l_qn[tid][0] = pq[2 * s.D];
     l_qn[tid][1] = pq[2 * s.D + 1];
   }
+  if constexpr (CO)
+    if (tid < 8) l_clo[tid] = cent_lo(s, tid);  // 4-bit: 8 centroids
   if constexpr (!NIB) {
     for (int i = tid; i < 128 * 32; i += blockDim.x) {
       _Float16 ch, cl;
@@ -146,40 +162,85 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
   f16v acc_m[RB][2];
   f16v acc_q[QJL ? RB : 1][2];
-  mfma_tile<NIB, QT, QJL, true>(s, planes, m, codes, n, q0, j0, stage, l_tab, ptab, acc_m, acc_q);
+  mfma_tile<NIB, QT, QJL, true, CO>(s, planes, m, codes, n, q0, j0, stage, l_tab, ptab, acc_m,
+                                    acc_q);
 
   // Epilogue: output element v of a 32x32 block sits at row
   // 8*(v/4) + 4*kg + (v%4), column lr. The per-code factors are formed once;
   // the expression is the reference's ip = (sigma*mse + scale*gamma*dot)*|x|*|q|,
-  // d = max(0, |q|^2 + |x|^2 - 2*ip).
+  // d = max(0, |q|^2 + |x|^2 - 2*ip). The coarse form then corrects its
+  // flagged pairs (bit 16 * (2 r + c) + v) and writes them again; every lane
+  // takes part, so no lane leaves early.
+  float fm[2], fq[2], xnv[2], xns[2];
+  bool jin[2];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     const long j = j0 + wave * 64 + 32 * c + lr;
-    if (j >= n) continue;
-    const float *meta = reinterpret_cast<const float *>(codes + j * (long)s.cs + s.packed);
+    jin[c] = j < n;
+    const float *meta =
+        reinterpret_cast<const float *>(codes + (jin[c] ? j : n - 1) * (long)s.cs + s.packed);
     const float xn = meta[0], gamma = meta[1], sigma = meta[2];
-    const float fm = sigma * s.mse_unscale;
-    const float fq = s.scale * gamma * kQjlUnscale;
-    const float xns = xn * xn;
+    fm[c] = sigma * s.mse_unscale;
+    fq[c] = s.scale * gamma * kQjlUnscale;
+    xnv[c] = xn;
+    xns[c] = xn * xn;
+  }
+  uint64_t flag = 0;
 #pragma unroll
-    for (int r = 0; r < RB; ++r)
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int row = 32 * r + 8 * (v >> 2) + 4 * kg + (v & 3);
         const long q = q0 + row;
-        if (q >= m) continue;
         float aq = 0.0f;
         if constexpr (QJL) aq = acc_q[r][c][v];
-        out[q * n + j] =
-            mfma_distance<QJL>(acc_m[r][c][v], aq, fm, fq, xn, xns, l_qn[row][0], l_qn[row][1]);
+        const float d = mfma_distance<QJL>(acc_m[r][c][v], aq, fm[c], fq[c], xnv[c], xns[c],
+                                           l_qn[row][0], l_qn[row][1]);
+        if (jin[c] && q < m) {
+          out[q * n + j0 + wave * 64 + 32 * c + lr] = d;
+          if constexpr (CO)
+            if (refine_flag(d, l_qn[row][0], xns[c])) flag |= 1ull << (16 * (2 * r + c) + v);
+        }
       }
+  if constexpr (CO) {
+    refine_flagged<QJL>(
+        s, l_clo, planes + split_planes_halves(s.D, m, Sh::NPL), codes, flag, lane,
+        [&](int b, int src, long &q, long &j) {
+          const int v = b & 15;
+          q = q0 + 32 * (b >> 5) + 8 * (v >> 2) + 4 * (src >> 5) + (v & 3);
+          j = j0 + wave * 64 + 32 * ((b >> 4) & 1) + (src & 31);
+        },
+        [&](int b, float dm, float dq) {
+          // this lane's sums of element b (a select chain: no indexing by b)
+          float am = 0.0f, aq = 0.0f;
+#pragma unroll
+          for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+              for (int v = 0; v < 16; ++v)
+                if (b == 16 * (2 * r + c) + v) {
+                  am = acc_m[r][c][v];
+                  if constexpr (QJL) aq = acc_q[r][c][v];
+                }
+          const int v = b & 15, c = (b >> 4) & 1;
+          const int row = 32 * (b >> 5) + 8 * (v >> 2) + 4 * kg + (v & 3);
+          out[(q0 + row) * n + j0 + wave * 64 + 32 * c + lr] = mfma_distance<QJL>(
+              am + dm, aq + dq, c ? fm[1] : fm[0], c ? fq[1] : fq[0], c ? xnv[1] : xnv[0],
+              c ? xns[1] : xns[0], l_qn[row][0], l_qn[row][1]);
+        });
   }
 }
 
 }  // namespace
 
 size_t split_planes_bytes(const SpaceDev &s, long m, bool qjl) {
-  return (size_t)split_planes_halves(s.D, m, split_planes(s.nib != 0, qjl)) * sizeof(_Float16);
+  const bool co = mfma_coarse(s);
+  long halves = split_planes_halves(s.D, m, split_planes(s.nib != 0, qjl, co));
+  if (co) halves += corr_halves(s.D, m);
+  return (size_t)halves * sizeof(_Float16);
 }
 
 hipError_t launch_split_queries(const SpaceDev &s, const float *prep, long m, _Float16 *planes,
@@ -188,8 +249,10 @@ hipError_t launch_split_queries(const SpaceDev &s, const float *prep, long m, _F
   const long threads = (m + 31) / 32 * (long)(s.D / 16) * 64;
   long blocks = (threads + 255) / 256;
   if (blocks > 65535) blocks = 65535;
+  const bool co = mfma_coarse(s);
+  const int npl = split_planes(s.nib != 0, qjl, co);
   hipLaunchKernelGGL(k_split_queries, dim3((unsigned)blocks), dim3(256), 0, st, s, prep, m,
-                     planes, split_planes(s.nib != 0, qjl));
+                     planes, npl, co ? planes + split_planes_halves(s.D, m, npl) : nullptr);
   return hipGetLastError();
 }
 
@@ -197,12 +260,15 @@ namespace {
 template <int QT, bool QJL>
 void launch_tile(const SpaceDev &s, dim3 grid, const _Float16 *planes, const float *prep, long m,
                  const uint8_t *codes, long n, float *out, hipStream_t st) {
-  if (s.nib)
-    hipLaunchKernelGGL((k_mfma<true, QT, QJL>), grid, dim3(256), 0, st, s, planes, prep, m,
+  if (mfma_coarse(s))
+    hipLaunchKernelGGL((k_mfma<true, QT, QJL, true>), grid, dim3(256), 0, st, s, planes, prep, m,
+                       codes, n, out);
+  else if (s.nib)
+    hipLaunchKernelGGL((k_mfma<true, QT, QJL, false>), grid, dim3(256), 0, st, s, planes, prep, m,
                        codes, n, out);
   else
-    hipLaunchKernelGGL((k_mfma<false, QT, QJL>), grid, dim3(256), 0, st, s, planes, prep, m,
-                       codes, n, out);
+    hipLaunchKernelGGL((k_mfma<false, QT, QJL, false>), grid, dim3(256), 0, st, s, planes, prep,
+                       m, codes, n, out);
 }
 }  // namespace
 

@@ -18,10 +18,12 @@ constexpr float kQjlUnscale = 1.0f / 1024.0f;   // 2^-10
 
 // Split query planes. Plane 0 = q_hi, 1 = q_lo, 2 = s_hi, 3 = s_lo (fp16 of
 // the 2^10-scaled q_rot and s_q, lo = fp16 of the remainder). The asymmetric
-// form of a nibble space reads all four; byte spaces skip s_lo (their QJL
-// term carries a small gamma); the symmetric MSE form reads q_hi and q_lo.
-__host__ __device__ constexpr int split_planes(bool nib, bool qjl) {
-  return qjl ? (nib ? 4 : 3) : 2;
+// form of a 2- or 3-bit nibble space reads all four; byte spaces and the
+// coarse form of 4-bit spaces (mfma_coarse) skip s_lo (a byte space's QJL
+// term carries a small gamma; the coarse form recomputes the pairs the
+// dropped terms matter for); the symmetric MSE form reads q_hi and q_lo.
+__host__ __device__ constexpr int split_planes(bool nib, bool qjl, bool co = false) {
+  return qjl ? (nib && !co ? 4 : 3) : 2;
 }
 // Layout: blocks of one MFMA A fragment (32 queries x 16 K slots, 1 KB),
 // [query block rb][k step ks][plane p][lane l][8 halves], ks = 0 .. D/16 - 1.
@@ -34,6 +36,15 @@ __host__ __device__ constexpr int split_planes(bool nib, bool qjl) {
 constexpr int kBlockHalves = 512;
 __host__ __device__ inline long split_planes_halves(int D, long m, int npl) {
   return (m + 31) / 32 * (long)(D / 16) * npl * kBlockHalves;
+}
+// The coarse form (mfma_coarse) adds, after the planes of the padded rows,
+// one correction row per query: for each group g of 8 coordinates, 16 halves
+// = q_hi of the group's coordinates, then s_lo, both in the planes' element
+// order (perm below). refine_block reads it for the terms the coarse passes
+// drop (q_hi * c_lo, s_lo * (+-1)).
+constexpr int kCorrHalves = 16;  // halves per 8-coordinate group
+__host__ __device__ inline long corr_halves(int D, long m) {
+  return (m + 31) / 32 * 32 * (long)(D / 8) * kCorrHalves;
 }
 __host__ __device__ inline int split_perm(bool nib, int e) {
   return nib ? (e < 4 ? 2 * e : 2 * (e - 4) + 1) : e;
@@ -77,9 +88,11 @@ __device__ __forceinline__ PermTab perm_tab(const SpaceDev &s) {
                  bytes[2][0], bytes[2][1], bytes[3][0], bytes[3][1]};
 }
 
-// 8 nibble units in one code word -> B fragments (c_hi, c_lo, +-1) in the
-// order (0, 2, 4, 6, 1, 3, 5, 7): bytes of the even (odd) coordinates' values
-// are looked up four at a time, then interleaved into fp16 pairs.
+// 8 nibble units in one code word -> B fragments (c_hi, c_lo unless !LO,
+// +-1) in the order (0, 2, 4, 6, 1, 3, 5, 7): bytes of the even (odd)
+// coordinates' values are looked up four at a time, then interleaved into
+// fp16 pairs.
+template <bool LO = true>
 __device__ __forceinline__ void decode_nib_word(uint32_t w, const PermTab &t, h8 &bh, h8 &bl,
                                                 h8 &bt) {
   const uint32_t ie = (w >> 1) & 0x07070707u;  // indices of coordinates 0, 2, 4, 6
@@ -96,7 +109,7 @@ __device__ __forceinline__ void decode_nib_word(uint32_t w, const PermTab &t, h8
     r[3] = __builtin_amdgcn_perm(Ho, Lo, 0x07030602u);
     bh = __builtin_bit_cast(h8, r);
   }
-  {
+  if constexpr (LO) {
     const uint32_t Le = __builtin_amdgcn_perm(t.ll_hi, t.ll_lo, ie);
     const uint32_t He = __builtin_amdgcn_perm(t.lh_hi, t.lh_lo, ie);
     const uint32_t Lo = __builtin_amdgcn_perm(t.ll_hi, t.ll_lo, io);

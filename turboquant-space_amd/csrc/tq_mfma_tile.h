@@ -17,10 +17,13 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // The same at 4-byte alignment (code rows are 4-byte aligned).
 typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
 
-template <bool NIB, int QT, bool QJL>
+// CO: the coarse pass set of 4-bit spaces (mfma_coarse): no s_lo plane and
+// no q_hi*c_lo pass; the caller recomputes the near pairs (refine_block).
+template <bool NIB, int QT, bool QJL, bool CO = false>
 struct MfmaShape {
+  static_assert(NIB || !CO, "the coarse pass set is for nibble codes");
   static constexpr int RB = QT / 32;                          // 32-row blocks per wavefront
-  static constexpr int NPL = split_planes(NIB, QJL);          // query planes read
+  static constexpr int NPL = split_planes(NIB, QJL, CO);      // query planes read
   static constexpr int NB = NPL * RB * 2;                     // 1 KB A blocks per chunk
   static constexpr int ALOADS = (NB * 64 + 255) / 256;        // 16-byte A loads per thread
   static constexpr int NC = NIB ? 1 : 2;                      // 16-byte code loads per thread
@@ -38,15 +41,15 @@ struct MfmaShape {
 // byte codes' centroid table (REP: replicated over 32 banks, decode8<true>)
 // and ptab the nibble codes' perm tables. Ends with a workgroup barrier after
 // the last chunk, so the caller may reuse the stage buffers.
-template <bool NIB, int QT, bool QJL, bool REP>
+template <bool NIB, int QT, bool QJL, bool REP, bool CO>
 __device__ __forceinline__ void mfma_tile(const SpaceDev &s, const _Float16 *__restrict__ planes,
                                           long m, const uint8_t *__restrict__ codes, long n,
                                           long q0, long j0,
-                                          uint8_t (*stage)[MfmaShape<NIB, QT, QJL>::STAGE],
+                                          uint8_t (*stage)[MfmaShape<NIB, QT, QJL, CO>::STAGE],
                                           const uint32_t *l_tab, const PermTab &ptab,
                                           f16v (&acc_m)[QT / 32][2],
                                           f16v (&acc_q)[QJL ? QT / 32 : 1][2]) {
-  using Sh = MfmaShape<NIB, QT, QJL>;
+  using Sh = MfmaShape<NIB, QT, QJL, CO>;
   constexpr int RB = Sh::RB, NPL = Sh::NPL, NB = Sh::NB, NC = Sh::NC;
   constexpr int ALOADS = Sh::ALOADS, kRing = Sh::RING;
   static_assert(QT == 32 || QT == 64, "query tile");
@@ -152,7 +155,7 @@ __device__ __forceinline__ void mfma_tile(const SpaceDev &s, const _Float16 *__r
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         if constexpr (NIB) {
-          decode_nib_word(cw[c][kb], ptab, bh[c], bl[c], bt[c]);
+          decode_nib_word<!CO>(cw[c][kb], ptab, bh[c], bl[c], bt[c]);
         } else {
           uint32_t u[8];
 #pragma unroll
@@ -171,7 +174,7 @@ __device__ __forceinline__ void mfma_tile(const SpaceDev &s, const _Float16 *__r
       if constexpr (QJL) { TQ_PASS(acc_q, 2, bt) }
       TQ_PASS(acc_m, 1, bh)
       if constexpr (NPL > 3) { TQ_PASS(acc_q, 3, bt) }
-      TQ_PASS(acc_m, 0, bl)
+      if constexpr (!CO) { TQ_PASS(acc_m, 0, bl) }
 #undef TQ_PASS
     }
   };
@@ -233,6 +236,144 @@ __device__ __forceinline__ float mfma_distance(float am, float aq, float fm, flo
     ip = am * fm * xn * qn;
   const float d = qns + xns - 2.0f * ip;
   return d > 0.0f ? d : 0.0f;
+}
+
+// Coarse form: a pair is corrected when its coarse distance is below
+// kRefineFrac of (|q|^2 + |x|^2). The terms the coarse pass set drops
+// (q_hi*c_lo, s_lo*(+-1)) move a distance by up to a few 1e-5 of that scale,
+// which is a few 1e-5 relative to every distance above the bar; the pairs
+// below it (a query's near codes, the ones a top-k decides on) get the
+// dropped terms added (refine_block), which makes them the full pass set's
+// value (only q_lo*c_lo, ~2^-22 relative, stays out).
+constexpr float kRefineFrac = 0.5f;
+// Corrections run kRefinePairs pairs at a time, kRefineLanes lanes each.
+constexpr int kRefineLanes = 8;
+constexpr int kRefinePairs = 64 / kRefineLanes;
+
+__device__ __forceinline__ bool refine_flag(float d, float qns, float xns) {
+  return d < kRefineFrac * (qns + xns);
+}
+
+// The terms the coarse passes drop for one (query, 4-bit code) pair, in the
+// MFMA sums' units: dm = sum_i q_hi[i] * c_lo[idx_i], dq = sum_i s_lo[i] *
+// (+-1)_i, from the query's correction row (tq_mfma_dev.h) and the code's
+// nibbles, by the kRefineLanes lanes of one eighth of a wavefront: lane
+// `sub` takes the 8-coordinate groups sub, sub + 8, ...; the partial sums
+// meet in a butterfly whose grouping is the same in every lane, so the
+// eight lanes return the same bits, and so does any other eighth given the
+// same pair. l_clo:
+// the space's 8 scaled centroid remainders c_lo in LDS. QJL = false: the
+// symmetric MSE form (no s term).
+template <bool QJL>
+__device__ __forceinline__ void refine_delta(const SpaceDev &s, const float *l_clo,
+                                          const _Float16 *__restrict__ crow,
+                                          const uint8_t *__restrict__ code, int sub, float &dm,
+                                          float &dq) {
+  float am = 0.0f, aq = 0.0f;
+  const int ng = s.D >> 3;
+  // Eight groups per step, all loads issued before the first use (a group
+  // past the end loads group `sub` again and is not added).
+  for (int g0 = sub; g0 < ng; g0 += 8 * kRefineLanes) {
+    uint32_t w[8];
+    h8 qh[8], sl[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int g = g0 + kRefineLanes * t < ng ? g0 + kRefineLanes * t : sub;
+      w[t] = *reinterpret_cast<const uint32_t *>(code + 4 * g);
+      qh[t] = *reinterpret_cast<const h8 *>(crow + g * kCorrHalves);
+      if constexpr (QJL) sl[t] = *reinterpret_cast<const h8 *>(crow + g * kCorrHalves + 8);
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      if (g0 + kRefineLanes * t >= ng) break;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const uint32_t u = (w[t] >> (4 * split_perm(true, e))) & 15u;  // element e's nibble
+        am = __builtin_fmaf((float)qh[t][e], l_clo[u >> 1], am);
+        if constexpr (QJL) aq += (u & 1u) ? (float)sl[t][e] : -(float)sl[t][e];
+      }
+    }
+  }
+#pragma unroll
+  for (int o = kRefineLanes / 2; o > 0; o >>= 1) {  // lane ^ o stays within the eighth
+    am += __shfl_xor(am, o);
+    if constexpr (QJL) aq += __shfl_xor(aq, o);
+  }
+  dm = am;
+  dq = aq;
+}
+
+// A wavefront's flagged pairs (bit b of `flag` in lane l: pair pos(b, l) =
+// (query row, code column), b = 16 * block + element) get the dropped terms
+// computed by refine_delta, up to kRefinePairs pairs at a time (one per
+// eighth of the wavefront, at most one per lane), taken in lane then bit
+// order; the
+// owning lane then calls finish(b, dm, dq) (b differs between lanes). `corr`
+// is the batch's correction rows. Every lane of the wavefront must call it.
+template <bool QJL, typename Pos, typename Finish>
+__device__ __forceinline__ void refine_flagged(const SpaceDev &s, const float *l_clo,
+                                               const _Float16 *__restrict__ corr,
+                                               const uint8_t *__restrict__ codes, uint64_t flag,
+                                               int lane, Pos &&pos, Finish &&finish) {
+  uint64_t any = __ballot(flag != 0u);
+  while (any) {
+    int src[kRefinePairs], bit[kRefinePairs];
+    bool ok[kRefinePairs];
+#pragma unroll
+    for (int p = 0; p < kRefinePairs; ++p) {
+      ok[p] = any != 0;
+      if (ok[p]) {
+        src[p] = __builtin_ctzll(any);
+        any &= any - 1;  // this lane is done for the round
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)flag, src[p]);
+        const uint32_t hi =
+            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(flag >> 32), src[p]);
+        bit[p] = __builtin_ctzll(((uint64_t)hi << 32) | lo);
+      } else {  // an idle eighth scores pair 0 again and keeps nothing
+        src[p] = src[0];
+        bit[p] = bit[0];
+      }
+    }
+    long q = 0, j = 0;
+#pragma unroll
+    for (int p = 0; p < kRefinePairs; ++p) {
+      long qp, jp;
+      pos(bit[p], src[p], qp, jp);
+      if (lane / kRefineLanes == p) {
+        q = qp;
+        j = jp;
+      }
+    }
+    float dm, dq;
+    refine_delta<QJL>(s, l_clo, corr + q * (long)(s.D / 8) * kCorrHalves, codes + j * (long)s.cs,
+                      lane % kRefineLanes, dm, dq);
+    int mb = -1;
+    float mdm = 0.0f, mdq = 0.0f;
+#pragma unroll
+    for (int p = 0; p < kRefinePairs; ++p) {
+      const int l0 = kRefineLanes * p;
+      const float pm = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dm), l0));
+      const float pq = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dq), l0));
+      if (ok[p] && lane == src[p]) {
+        mb = bit[p];
+        mdm = pm;
+        mdq = pq;
+      }
+    }
+    if (mb >= 0) {
+      flag &= flag - 1u;  // bit mb is this lane's lowest
+      finish(mb, mdm, mdq);
+    }
+    any = __ballot(flag != 0u);
+  }
+}
+
+// The scaled centroid remainders of a 4-bit space (c * cscale - c_hi, the
+// part of each centroid the coarse passes leave out), for l_clo.
+__device__ __forceinline__ float cent_lo(const SpaceDev &s, int k) {
+  _Float16 ch, cl;
+  cent_h(s, k, ch, cl);
+  return s.cent[k] * s.cscale - (float)ch;
 }
 
 }  // namespace mfma

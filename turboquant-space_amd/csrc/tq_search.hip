@@ -124,13 +124,14 @@ struct SearchShape {
 // [w][0..K-1][l], ascending, empty entries (inf, ~0) last) is read into
 // registers only when one of the wavefront's distances beats a list's kk-th
 // entry (kk = the k asked for), and written back after.
-template <bool NIB, int QR, int K>
+template <bool NIB, bool CO, int QR, int K>
 __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, long j0, int kk,
                                             uint8_t *lds, const uint32_t *l_tab,
                                             const PermTab &ptab, const float (*l_qn)[2],
-                                            float (*l_ld)[K][64], uint32_t (*l_li)[K][64]) {
+                                            const float *l_clo, float (*l_ld)[K][64],
+                                            uint32_t (*l_li)[K][64]) {
   constexpr int QT = 32 * QR;
-  using Sh = MfmaShape<NIB, QT, true>;
+  using Sh = MfmaShape<NIB, QT, true, CO>;
   const SpaceDev &s = job.s;
   const long m = job.m, n = job.n;
   const int tid = threadIdx.x;
@@ -142,8 +143,8 @@ __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, lo
 
   f16v acc_m[QR][2], acc_q[QR][2];
   __syncthreads();  // the previous tile's transpose reads are done with the stage area
-  mfma_tile<NIB, QT, true, false>(s, job.planes, m, job.codes, n, q0, j0, stage, l_tab, ptab,
-                                  acc_m, acc_q);
+  mfma_tile<NIB, QT, true, false, CO>(s, job.planes, m, job.codes, n, q0, j0, stage, l_tab, ptab,
+                                      acc_m, acc_q);
 
   const long jbase = j0 + wave * 64;
   float fm[2], fq[2], xnv[2], xns[2];
@@ -162,16 +163,48 @@ __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, lo
   const long cbase = jbase + 32 * h;
 #pragma unroll
   for (int r = 0; r < QR; ++r) {
+    // k_mfma's epilogue: the same distances, the coarse form's flagged pairs
+    // (bit 16 c + v) corrected the same way.
+    uint64_t flag = 0;
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < 2; ++c) {
+      const bool jin = jbase + 32 * c + lr < n;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int row = 8 * (v >> 2) + 4 * kg + (v & 3);
         const int qrow = 32 * r + row;
-        t_lds[wave][row][32 * c + lr] = mfma_distance<true>(acc_m[r][c][v], acc_q[r][c][v], fm[c],
-                                                            fq[c], xnv[c], xns[c], l_qn[qrow][0],
-                                                            l_qn[qrow][1]);
+        const float d = mfma_distance<true>(acc_m[r][c][v], acc_q[r][c][v], fm[c], fq[c], xnv[c],
+                                            xns[c], l_qn[qrow][0], l_qn[qrow][1]);
+        t_lds[wave][row][32 * c + lr] = d;
+        if constexpr (CO)
+          if (jin && q0 + qrow < m && refine_flag(d, l_qn[qrow][0], xns[c]))
+            flag |= 1ull << (16 * c + v);
       }
+    }
+    if constexpr (CO)
+      refine_flagged<true>(
+          s, l_clo, job.planes + split_planes_halves(s.D, m, Sh::NPL), job.codes, flag, lane,
+          [&](int b, int src, long &q, long &j) {
+            const int v = b & 15;
+            q = q0 + 32 * r + 8 * (v >> 2) + 4 * (src >> 5) + (v & 3);
+            j = jbase + 32 * (b >> 4) + (src & 31);
+          },
+          [&](int b, float dm, float dq) {
+            float am = 0.0f, aq = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+              for (int v = 0; v < 16; ++v)
+                if (b == 16 * c + v) {
+                  am = acc_m[r][c][v];
+                  aq = acc_q[r][c][v];
+                }
+            const int v = b & 15, c = b >> 4;
+            const int row = 8 * (v >> 2) + 4 * kg + (v & 3);
+            t_lds[wave][row][32 * c + lr] = mfma_distance<true>(
+                am + dm, aq + dq, c ? fm[1] : fm[0], c ? fq[1] : fq[0], c ? xnv[1] : xnv[0],
+                c ? xns[1] : xns[0], l_qn[32 * r + row][0], l_qn[32 * r + row][1]);
+          });
     __syncthreads();
     const bool mine = QR == 1 || (lane >> 5) == r;
     const float *rowp = &t_lds[wave][lane & 31][32 * h];
@@ -236,6 +269,7 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
   __shared__ __attribute__((aligned(16))) uint8_t lds[Sh::LDS];
   __shared__ uint32_t l_tab[128];
   __shared__ float l_qn[QT][2];
+  __shared__ float l_clo[8];  // 4-bit jobs: the centroid remainders, for refine_block
   __shared__ float l_ld[4][K][64];
   __shared__ uint32_t l_li[4][K][64];
   __shared__ int s_item;
@@ -257,6 +291,7 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
     const SearchJobDev &job = jobs[item.job];
     const SpaceDev &s = job.s;
     const bool nib = s.nib != 0;
+    const bool coarse = mfma_coarse(s);
     const long m = job.m;
     const long q0 = (long)item.qtile * QT;
 
@@ -266,6 +301,7 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
         cent_h(s, i, ch, cl);
         l_tab[i] = h_bits(ch) | (h_bits(cl) << 16);
       }
+    if (coarse && tid < 8) l_clo[tid] = cent_lo(s, tid);
     if (tid < QT) {
       const long q = q0 + tid;
       const float *pq = job.prep + (q < m ? q : 0L) * prep_stride(s.D);
@@ -283,10 +319,15 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
     // orders the tables above before its first use of them.)
     for (int tile = item.t0; tile < item.t1; ++tile) {
       const long j0 = (long)tile * kSN;
-      if (nib)
-        search_tile<true, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_ld, l_li);
+      if (coarse)
+        search_tile<true, true, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_ld,
+                                       l_li);
+      else if (nib)
+        search_tile<true, false, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_ld,
+                                        l_li);
       else
-        search_tile<false, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_ld, l_li);
+        search_tile<false, false, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_ld,
+                                         l_li);
     }
 
     // This wavefront's lists for the item.
