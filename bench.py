@@ -139,10 +139,21 @@ def bench_batched(torch, capi, stream, sh, reps=5):
                               "queries_per_s": total_q / (ms / 1e3),
                               "tflops": flops / (ms / 1e3) / 1e12,
                               "form_of_costliest_collection": forms.get(form, form),
-                              "tuned_fused_knobs": {"query_tile": qt, "items_per_wg": ipw}}
+                              "tuned_fused_knobs": {"query_tile": qt, "items_per_wg": ipw},
+                              "tuned_knobs_costliest_batch": plan.tuned}
+    # The tuner's table entry of each collection (one batch per collection:
+    # the plan coalesces the jobs aimed at it).
+    per_coll = []
+    for t, (sp, _, _) in enumerate(colls):
+        mt = sum(m for tt, m in batches if tt == t)
+        per_coll.append({"dim": sp.dim, "bits": BATCH_COLLS[t][1], "m": mt,
+                         "knobs": capi.tuning_lookup(sp.D, BATCH_COLLS[t][1], mt)})
+    out["tuned_mfma_plan"]["tuned_per_collection"] = per_coll
     ref_ids = [j[6].clone() for j in jobs]
     plan.close()
     log(f"batched: tuned MFMA plan graph {ms:.2f} ms ({dists / (ms / 1e3) / 1e9:.1f} G dist/s)")
+    for pc in per_coll:
+        log(f"batched: tuned knobs dim {pc['dim']} {pc['bits']}-bit m {pc['m']}: {pc['knobs']}")
     fplan = capi.SearchPlan(jobs, 10, capi.TQ_MODE_MFMA, form=capi.TQ_FORM_FUSED)
     items, wgs = fplan.stats()
     fplan.run_graph(stream=sh)

@@ -220,12 +220,19 @@ typedef struct tq_search_job {
 /* Knobs of TQ_MODE_MFMA plans. 0 = automatic: the autotuner's table for the
  * job's (padded dim, bits, m) bucket if it has an entry
  * (tq_search_plan_tune), else the built-in rule (blocked; fused where a batch
- * has more than 65535 queries or no codes). query_tile and items_per_wg
- * shape the fused form's work queue. */
+ * has more than 65535 queries or no codes). query_tile, items_per_wg and
+ * wg_per_cu shape the fused form's work queue and persistent grid;
+ * blocked_query_tile and block_mb the blocked form's MFMA tiles and distance
+ * blocks. No knob changes a result: every distance is a function of its
+ * (query, code) pair and ties go by id. */
 typedef struct tq_search_tuning {
-  int32_t query_tile;   /* 32 or 64 queries per work item, or 0 */
-  int32_t items_per_wg; /* work items per persistent workgroup (load balance), or 0 */
+  int32_t query_tile;   /* fused form: 32 or 64 queries per work item, or 0 */
+  int32_t items_per_wg; /* fused form: work items per persistent workgroup (load balance), or 0 */
   int32_t form;         /* TQ_FORM_*, or 0 */
+  int32_t wg_per_cu;    /* fused form: persistent workgroups per compute unit, 1 or 2, or 0 */
+  int32_t blocked_query_tile; /* blocked form: MFMA query tile, 32 or 64, or 0 (32 up to
+                                 32 queries, else 64) */
+  int32_t block_mb;     /* blocked form: distance block budget in MiB, 16..256, or 0 (256) */
 } tq_search_tuning;
 
 typedef struct tq_search_plan_s *tq_search_plan;
@@ -253,12 +260,17 @@ int tq_search_plan_run(tq_search_plan p, void *stream);
 int tq_search_plan_run_graph(tq_search_plan p, void *stream);
 /* Times the candidate knobs for each (padded dim, bits, m) bucket of the
  * plan's jobs on the plan's own buffers (overwriting the outputs): the
- * blocked form and the fused form with query_tile in {32, 64} x
- * items_per_wg in {1, 2, 4, 8}. Records the fastest form and the fastest
- * fused knobs in the process-wide table, and rebuilds the plan with them.
- * Synchronous. `chosen` (may be NULL) receives the plan's resulting knobs
- * (query_tile and form of its costliest batch). */
+ * fused form with query_tile in {32, 64} x items_per_wg in {1, 2, 4, 8} x
+ * wg_per_cu in {1, 2}, and the blocked form with blocked_query_tile in
+ * {32, 64} x block_mb in {64, 128, 256}. Records the fastest form and the
+ * fastest knobs of each form in the process-wide table, and rebuilds the
+ * plan with them. Synchronous. `chosen` (may be NULL) receives the plan's
+ * resulting knobs (those of its costliest batch). */
 int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen);
+/* The autotuner's table entry for (padded dim, bits, m bucket of m), if any:
+ * TQ_OK and *out filled, or TQ_EINVAL when the bucket has not been tuned.
+ * No counterpart in the reference (it has no search or tuner). */
+int tq_search_tuning_lookup(int32_t padded_dim, int32_t bits, uint64_t m, tq_search_tuning *out);
 /* Work items and persistent workgroups of the plan (either may be NULL). */
 int tq_search_plan_stats(tq_search_plan p, uint64_t *items, uint64_t *workgroups);
 int tq_search_plan_destroy(tq_search_plan p);

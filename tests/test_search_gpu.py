@@ -259,6 +259,9 @@ def test_tuned_plan_gives_the_same_results(gpu, torch_cuda):
     before = (od.clone(), oi.clone())
     qt, ipw, form = plan.tune()
     assert qt in (32, 64) and ipw in (1, 2, 4, 8) and form in (capi.TQ_FORM_FUSED, capi.TQ_FORM_BLOCKED)
+    t = plan.tuned
+    assert t["wg_per_cu"] in (1, 2) and t["blocked_query_tile"] == 32  # 20 queries: one 32-row tile
+    assert t["block_mb"] in (64, 128, 256)
     od.fill_(0)
     plan.run()
     torch.cuda.synchronize()
@@ -274,6 +277,34 @@ def test_tuned_plan_gives_the_same_results(gpu, torch_cuda):
     assert torch.equal(before[0], od) and torch.equal(before[1], oi)
     assert fused.stats()[0] > 0
     fused.close()
+
+
+@pytest.mark.parametrize("knobs", [
+    dict(form=2, blocked_query_tile=32, block_mb=16),   # 32-row MFMA tiles, 16 MiB blocks
+    dict(form=2, blocked_query_tile=64, block_mb=64),
+    dict(form=1, query_tile=64, items_per_wg=2, wg_per_cu=1),
+    dict(form=1, query_tile=32, items_per_wg=8, wg_per_cu=2),
+])
+def test_every_knob_gives_the_same_bits(gpu, torch_cuda, knobs):
+    # Each autotuner knob changes only how the work is cut: the top-k of a
+    # plan built with it equals the default plan's bit for bit (70 queries:
+    # three 32-row or two 64-row tiles; 130,000 codes: three chunks of a
+    # 16 MiB block, 59,904 columns each).
+    torch, capi = torch_cuda, gpu
+    say(f"[gpu] knobs {knobs}")
+    knobs = dict(knobs)
+    knobs["form"] = capi.TQ_FORM_FUSED if knobs["form"] == 1 else capi.TQ_FORM_BLOCKED
+    c = Coll(torch, capi, 768, 4, 130000, 43)
+    _, dQ = c.queries(torch, 70)
+    res = []
+    for kw in ({}, knobs):
+        od, oi = outputs(torch, 70, 10)
+        plan = capi.SearchPlan([(c.sp, dQ, 70, c.codes, c.n, od, oi)], 10, capi.TQ_MODE_MFMA, **kw)
+        plan.run()
+        torch.cuda.synchronize()
+        res.append((od.cpu(), oi.cpu()))
+        plan.close()
+    assert torch.equal(res[0][0], res[1][0]) and torch.equal(res[0][1], res[1][1])
 
 
 def test_k_above_n_and_one_shot_search(gpu, torch_cuda):
@@ -336,6 +367,10 @@ def test_search_rejects_bad_arguments(gpu, torch_cuda):
         capi.SearchPlan([(c.sp, dQ, 2, c.codes, 10, od17, oi17)], 17, form=capi.TQ_FORM_FUSED)
     with pytest.raises(capi.TQError, match="query_tile"):
         capi.SearchPlan([(c.sp, dQ, 2, c.codes, 10, od, oi)], 10, query_tile=48)
+    for bad in (dict(wg_per_cu=3), dict(blocked_query_tile=16), dict(block_mb=8),
+                dict(block_mb=512)):
+        with pytest.raises(capi.TQError):
+            capi.SearchPlan([(c.sp, dQ, 2, c.codes, 10, od, oi)], 10, **bad)
     with pytest.raises(capi.TQError, match="NULL"):
         capi.SearchPlan([(c.sp, None, 2, c.codes, 10, od, oi)], 10)
 

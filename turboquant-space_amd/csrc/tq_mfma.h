@@ -17,7 +17,9 @@ hipError_t launch_split_queries(const SpaceDev &s, const float *prep, long m,
                                 _Float16 *planes, hipStream_t st, bool qjl = true);
 // The MFMA distances [m][n] of queries already split into `planes` (written
 // by launch_split_queries with the same qjl, m <= 65535 * 64), no allocation.
+// qt: queries per workgroup tile, 32 or 64 (0: 32 for m <= 32, else 64);
+// every element is the same whichever tile computes it.
 hipError_t launch_mfma_planes(const SpaceDev &s, const _Float16 *planes, const float *prep, long m,
                               const uint8_t *codes, long n, float *out, hipStream_t st,
-                              bool qjl = true);
+                              bool qjl = true, int qt = 0);
 }  // namespace tq

@@ -273,12 +273,16 @@ void launch_tile(const SpaceDev &s, dim3 grid, const _Float16 *planes, const flo
 }  // namespace
 
 hipError_t launch_mfma_planes(const SpaceDev &s, const _Float16 *planes, const float *prep, long m,
-                              const uint8_t *codes, long n, float *out, hipStream_t st, bool qjl) {
+                              const uint8_t *codes, long n, float *out, hipStream_t st, bool qjl,
+                              int qt) {
   if (s.D < kKC || m > 65535L * 64) return hipErrorNotSupported;
   if (m <= 0 || n <= 0) return hipSuccess;
   const long gx = (n + kTN - 1) / kTN;
-  if (m <= 32) {  // one 32-query tile: half the matrix work of a 64-query one
-    const dim3 grid((unsigned)gx);
+  if (qt == 0) qt = m <= 32 ? 32 : 64;  // one 32-query tile: half the matrix work of a 64-query one
+  if (qt == 32) {
+    const long tiles = gx * ((m + 31) / 32);
+    if (tiles > 0xFFFFFFFFL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)tiles);
     if (qjl)
       launch_tile<32, true>(s, grid, planes, prep, m, codes, n, out, st);
     else

@@ -30,8 +30,8 @@ struct MfmaShape {
   static constexpr int A_BYTES = NB * 1024;
   static constexpr int STAGE = A_BYTES + 4 * NC * 1024;
   // Chunks whose loads are in the register ring: three where the registers
-  // allow it, two for the 64-query tiles with QJL passes.
-  static constexpr int RING = (QT == 64 && QJL) ? 2 : 3;
+  // allow it, two for the 64-query tiles with four QJL-pass planes.
+  static constexpr int RING = (QT == 64 && QJL && !CO) ? 2 : 3;
 };
 
 // acc_m / acc_q of queries [q0, q0 + QT) x codes [j0, j0 + 256) of one

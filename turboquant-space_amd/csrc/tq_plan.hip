@@ -113,7 +113,9 @@ struct PlanBatch {
   long row0 = 0;              // first row in the plan-owned outputs
   std::vector<int> jobs;      // indices into plan->jobs, in row order
   std::vector<long> job_row;  // first row of each job within the batch
-  int qr = 1;                 // query tile: 32 * qr
+  int qr = 1;                 // fused query tile: 32 * qr
+  int bqt = 0;                // blocked: k_mfma query tile (0: by m)
+  size_t block_bytes = 0;     // blocked: distance block budget
   bool blocked = false;       // MFMA plans: k_mfma blocks + row top-k instead of k_search
   // The plain fp32 scan in row blocks + row top-k: every batch of an FP32
   // plan, and batches of padded dim below 32 (under the MFMA tile's
@@ -223,6 +225,15 @@ tq_search_tuning knobs_for(const tq_search_plan_s *p, const PlanBatch &b) {
                    : have                 ? tab.items_per_wg
                                           : tq::kDefaultItemsPerWg;
   t.form = p->forced.form ? p->forced.form : have && tab.form ? tab.form : TQ_FORM_BLOCKED;
+  t.wg_per_cu = p->forced.wg_per_cu ? p->forced.wg_per_cu
+                : have && tab.wg_per_cu ? tab.wg_per_cu
+                                        : 2;
+  t.blocked_query_tile = p->forced.blocked_query_tile ? p->forced.blocked_query_tile
+                         : have                       ? tab.blocked_query_tile
+                                                      : 0;
+  t.block_mb = p->forced.block_mb ? p->forced.block_mb
+               : have && tab.block_mb ? tab.block_mb
+                                      : (int)(tq::kMfmaBlockBytes >> 20);
   return t;
 }
 
@@ -241,12 +252,15 @@ int layout(tq_search_plan_s *p) {
   p->grid_qr[0] = p->grid_qr[1] = 0;
   p->gather_max = p->scatter_max = 0;
 
-  // Plan-level items per workgroup: the knob of the costliest batch.
-  int ipw = tq::kDefaultItemsPerWg;
+  // Plan-level items per workgroup and persistent workgroups per CU: the
+  // knobs of the costliest batch.
+  int ipw = tq::kDefaultItemsPerWg, wpc = 2;
   double worst = -1.0;
   for (PlanBatch &b : p->batches) {
     const tq_search_tuning t = knobs_for(p, b);
     b.qr = t.query_tile == 64 ? 2 : 1;
+    b.bqt = t.blocked_query_tile;
+    b.block_bytes = (size_t)t.block_mb << 20;
     b.plain = p->mode == TQ_MODE_FP32 || b.space->info.padded_dim < 32;
     // k above the fused form's list length always runs blocked (plan_new
     // refuses plans where that is not possible).
@@ -257,9 +271,11 @@ int layout(tq_search_plan_s *p) {
     if (cost > worst) {
       worst = cost;
       ipw = std::max(1, (int)t.items_per_wg);
+      wpc = t.wg_per_cu == 1 ? 1 : 2;
     }
   }
   p->used.items_per_wg = ipw;
+  p->used.wg_per_cu = wpc;
   p->used.query_tile = 0;
 
   // Items per query-tile kind, sized so each kind's queue holds about
@@ -280,7 +296,7 @@ int layout(tq_search_plan_s *p) {
         const long ntiles = (b.n + ntile - 1) / ntile;
         total += (double)qtiles * ntiles * b.space->info.padded_dim * kind;
       }
-      const long wg = (long)p->cus * 2;  // two workgroups per CU for both query tiles
+      const long wg = (long)p->cus * wpc;  // persistent workgroups for both query tiles
       const double target = total / (double)(wg * ipw);
       for (size_t bi = 0; bi < p->batches.size(); ++bi) {
         const PlanBatch &b = p->batches[bi];
@@ -330,7 +346,7 @@ int layout(tq_search_plan_s *p) {
     p->h_order = ord;
     for (const Pending &pi : pend) p->nitems_qr[pi.kind - 1]++;
     for (int kind = 1; kind <= 2; ++kind) {
-      const long wg = (long)p->cus * 2;  // two workgroups per CU for both query tiles
+      const long wg = (long)p->cus * wpc;
       p->grid_qr[kind - 1] = (int)std::min<long>(wg, p->nitems_qr[kind - 1]);
     }
   }
@@ -367,7 +383,7 @@ int layout(tq_search_plan_s *p) {
     if (!b.plain) {
       p->o_planes[bi] = take(tq::split_planes_bytes(b.space->dev, (long)b.m));
       if (b.blocked) {
-        long nc = (long)(tq::kMfmaBlockBytes / (sizeof(float) * (size_t)b.m));
+        long nc = (long)(b.block_bytes / (sizeof(float) * (size_t)b.m));
         nc = std::max(4096L, nc - nc % tq::kSearchNTile);
         b.nc = std::min(nc, b.n);
         b.lists = 1 + tq::topk_rows_segments(b.m, b.nc);
@@ -530,7 +546,7 @@ int enqueue(tq_search_plan_s *p, hipStream_t st) {
         const long l1 = 1 + tq::topk_rows_segments(b.m, nc);
         const bool last = c0 + nc >= b.n;
         if ((e = tq::launch_mfma_planes(b.space->dev, b.planes, b.prep, b.m, b.codes + c0 * cs, nc,
-                                        p->dist, st)) != hipSuccess ||
+                                        p->dist, st, true, b.bqt)) != hipSuccess ||
             (e = tq::launch_topk_rows_chunk(p->dist, b.m, nc, b.lists, 1, (uint32_t)c0, p->k,
                                             c0 > 0, p->seg_d, p->seg_i, st)) != hipSuccess ||
             (e = tq::launch_topk_rows_fold(b.m, b.lists, c0 == 0 ? 1 : 0, l1, p->k, p->seg_d,
@@ -594,6 +610,13 @@ int plan_new(const tq_search_job *jobs, uint64_t njobs, int k, int mode,
     if (tuning->form != TQ_FORM_AUTO && tuning->form != TQ_FORM_FUSED &&
         tuning->form != TQ_FORM_BLOCKED)
       return fail(TQ_EINVAL, "%s: form must be 0, TQ_FORM_FUSED or TQ_FORM_BLOCKED", what);
+    if (tuning->wg_per_cu < 0 || tuning->wg_per_cu > 2)
+      return fail(TQ_EINVAL, "%s: wg_per_cu must be 0, 1 or 2", what);
+    if (tuning->blocked_query_tile != 0 && tuning->blocked_query_tile != 32 &&
+        tuning->blocked_query_tile != 64)
+      return fail(TQ_EINVAL, "%s: blocked_query_tile must be 0, 32 or 64", what);
+    if (tuning->block_mb != 0 && (tuning->block_mb < 16 || tuning->block_mb > 256))
+      return fail(TQ_EINVAL, "%s: block_mb must be 0 or in [16, 256]", what);
   }
   int device = -1;
   for (uint64_t i = 0; i < njobs; ++i) {
@@ -744,6 +767,14 @@ int tq_search_plan_run_graph(tq_search_plan p, void *stream) {
   return TQ_OK;
 }
 
+int tq_search_tuning_lookup(int32_t padded_dim, int32_t bits, uint64_t m, tq_search_tuning *out) {
+  if (!out) return fail(TQ_EINVAL, "tq_search_tuning_lookup: out is NULL");
+  if (!tq::tune_lookup(padded_dim, bits, (long)m, out))
+    return fail(TQ_EINVAL, "tq_search_tuning_lookup: (%d, %d, m = %llu) has not been tuned",
+                padded_dim, bits, (unsigned long long)m);
+  return TQ_OK;
+}
+
 int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen) {
   const char *what = "tq_search_plan_tune";
   if (!p) return fail(TQ_EINVAL, "%s: NULL plan", what);
@@ -765,18 +796,25 @@ int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen
       if (b.m == 0 || b.n == 0 || b.plain) continue;  // plain batches have no knobs
       std::vector<tq_search_job> sub;
       for (int ji : b.jobs) sub.push_back(p->jobs[ji]);
-      // Candidates: the fused form's knobs, then the blocked form (when the
-      // batch allows it). The table keeps the fastest fused knobs and the
-      // fastest form.
-      tq_search_tuning best{};
-      float best_ms = 0.0f, best_form_ms = 0.0f;
+      // Candidates: the fused form's knobs (query tile, items per
+      // workgroup, persistent workgroups per CU), then the blocked form's
+      // (MFMA query tile, distance block budget) when the batch allows it.
+      // The table keeps the fastest knobs of each form and the fastest form.
+      tq_search_tuning best{}, best_blk{};
+      float best_ms = 0.0f, best_blk_ms = 0.0f, best_form_ms = 0.0f;
       int best_form = TQ_FORM_FUSED;
       std::vector<tq_search_tuning> cands;
       if (p->k <= tq::kSearchMaxK)
         for (int qt : {32, 64})
-          for (int ipw : {1, 2, 4, 8}) cands.push_back(tq_search_tuning{qt, ipw, TQ_FORM_FUSED});
+          for (int ipw : {1, 2, 4, 8})
+            for (int wpc : {1, 2})
+              cands.push_back(tq_search_tuning{qt, ipw, TQ_FORM_FUSED, wpc, 0, 0});
       if (b.space->info.padded_dim >= 32 && tq::plan_blocked(b.n, b.m))
-        cands.push_back(tq_search_tuning{0, 0, TQ_FORM_BLOCKED});
+        for (int bqt : {32, 64}) {
+          if (bqt == 64 && b.m <= 32) continue;  // one 32-row tile covers the batch
+          for (int mb : {64, 128, 256})
+            cands.push_back(tq_search_tuning{0, 0, TQ_FORM_BLOCKED, 0, bqt, mb});
+        }
       for (size_t ci = 0; ci < cands.size(); ++ci) {
         const tq_search_tuning &cand = cands[ci];
         tq_search_plan sp = nullptr;
@@ -797,6 +835,11 @@ int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen
           best = cand;
           best_ms = run_best;
         }
+        if (cand.form == TQ_FORM_BLOCKED &&
+            (best_blk.blocked_query_tile == 0 || run_best < best_blk_ms)) {
+          best_blk = cand;
+          best_blk_ms = run_best;
+        }
         if (ci == 0 || run_best < best_form_ms) {
           best_form = cand.form;
           best_form_ms = run_best;
@@ -804,6 +847,8 @@ int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen
       }
       if (rc == TQ_OK) {
         best.form = best_form;
+        best.blocked_query_tile = best_blk.blocked_query_tile;
+        best.block_mb = best_blk.block_mb;
         tq::tune_store((int)b.space->info.padded_dim, b.space->info.bits_per_coord, b.m, best);
       }
     }
@@ -815,7 +860,7 @@ int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen
   }
   if (chosen) {
     *chosen = p->used;
-    // The query tile and form of the costliest batch.
+    // The knobs of the costliest batch.
     double worst = -1.0;
     for (const PlanBatch &b : p->batches) {
       const double cost = (double)b.m * (double)b.n * (double)b.space->info.padded_dim;
@@ -823,6 +868,8 @@ int tq_search_plan_tune(tq_search_plan p, void *stream, tq_search_tuning *chosen
         worst = cost;
         chosen->query_tile = 32 * b.qr;
         chosen->form = b.blocked ? TQ_FORM_BLOCKED : TQ_FORM_FUSED;
+        chosen->blocked_query_tile = b.bqt ? b.bqt : (b.m <= 32 ? 32 : 64);
+        chosen->block_mb = (int)(b.block_bytes >> 20);
       }
     }
   }

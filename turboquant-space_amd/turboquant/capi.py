@@ -34,7 +34,8 @@ EXPORTED = [
     "tq_distance_m_to_n_symmetric", "tq_distance_m_to_n_symmetric_ex", "tq_prepare_symmetric",
     "tq_distance_symmetric_prepared", "tq_rotated_coords_batch", "tq_encode_graph_create",
     "tq_encode_graph_run", "tq_encode_graph_destroy", "tq_search_plan_create", "tq_search_plan_run",
-    "tq_search_plan_run_graph", "tq_search_plan_tune", "tq_search_plan_stats",
+    "tq_search_plan_run_graph", "tq_search_plan_tune", "tq_search_tuning_lookup",
+    "tq_search_plan_stats",
     "tq_search_plan_destroy", "tq_search", "tq_topk_rows", "tq_server_create",
     "tq_server_destroy", "tq_server_add_collection", "tq_server_submit_append",
     "tq_server_submit_search", "tq_server_wait", "tq_server_collection", "tq_server_load_codes", "tq_server_read_codes",
@@ -73,7 +74,8 @@ class SearchJob(ctypes.Structure):
 
 class SearchTuning(ctypes.Structure):
     _fields_ = [("query_tile", ctypes.c_int32), ("items_per_wg", ctypes.c_int32),
-                ("form", ctypes.c_int32)]
+                ("form", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
+                ("blocked_query_tile", ctypes.c_int32), ("block_mb", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -153,6 +155,8 @@ def lib():
         L.tq_search_plan_run_graph.argtypes = [P, P]
         L.tq_search_plan_tune.restype = I
         L.tq_search_plan_tune.argtypes = [P, P, TP]
+        L.tq_search_tuning_lookup.restype = I
+        L.tq_search_tuning_lookup.argtypes = [ctypes.c_int32, ctypes.c_int32, U64, TP]
         L.tq_search_plan_stats.restype = I
         L.tq_search_plan_stats.argtypes = [P, ctypes.POINTER(U64), ctypes.POINTER(U64)]
         L.tq_search_plan_destroy.restype = I
@@ -308,14 +312,23 @@ class EncodeGraph:
             pass
 
 
+def tuning_lookup(padded_dim, bits, m):
+    """The autotuner's knobs for (padded dim, bits, m bucket), or None."""
+    t = SearchTuning()
+    if lib().tq_search_tuning_lookup(padded_dim, bits, m, ctypes.byref(t)) != TQ_OK:
+        return None
+    return {f: int(getattr(t, f)) for f, _ in SearchTuning._fields_}
+
+
 class SearchPlan:
     """Owns a tq_search_plan over device-resident job buffers."""
 
     def __init__(self, jobs, k, mode=TQ_MODE_MFMA, query_tile=0, items_per_wg=0,
-                 form=TQ_FORM_AUTO):
+                 form=TQ_FORM_AUTO, wg_per_cu=0, blocked_query_tile=0, block_mb=0):
         self._jobs = make_jobs(jobs)
         self._keep = jobs  # the spaces must outlive the plan
-        tun = SearchTuning(query_tile, items_per_wg, form)
+        tun = SearchTuning(query_tile, items_per_wg, form, wg_per_cu, blocked_query_tile,
+                           block_mb)
         h = ctypes.c_void_p()
         check(lib().tq_search_plan_create(self._jobs, len(jobs), k, mode, ctypes.byref(tun),
                                           ctypes.byref(h)))
@@ -329,8 +342,12 @@ class SearchPlan:
         check(lib().tq_search_plan_run_graph(self.h, stream))
 
     def tune(self, stream=None):
+        """Autotunes every batch of the plan and rebuilds it; returns the
+        fused query tile, items per workgroup and form of the costliest batch
+        (all knobs of that batch are in `tuned` afterwards)."""
         t = SearchTuning()
         check(lib().tq_search_plan_tune(self.h, stream, ctypes.byref(t)))
+        self.tuned = {f: int(getattr(t, f)) for f, _ in SearchTuning._fields_}
         return int(t.query_tile), int(t.items_per_wg), int(t.form)
 
     def stats(self):
