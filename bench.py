@@ -237,7 +237,7 @@ def bench_serving(torch, capi, sh, qps_by_mode, seconds, n_per_coll=SERVE_N, see
             c, off = c + 1, 0
     torch.cuda.synchronize()
     rng = np.random.default_rng(seed)
-    longest = max(seconds * max(q) for q in qps_by_mode.values())
+    longest = max(seconds * max(q) for q in qps_by_mode.values() if q)
     horizon = int(max(longest * 1.3, 50))
     # Request pool, drawn once: kinds and sizes; search queries are drawn per
     # request, append payloads from a pool of 16 (host memory stays bounded).
@@ -333,6 +333,8 @@ def bench_serving(torch, capi, sh, qps_by_mode, seconds, n_per_coll=SERVE_N, see
             "failed": st["failed"] - st0["failed"]}
 
     for mode_name, points in qps_by_mode.items():
+        if not points:
+            continue  # this form's curve was not asked for
         mode = capi.TQ_MODE_MFMA if mode_name == "mfma" else capi.TQ_MODE_FP32
         # Capacity for every append the sweep can submit (points share the
         # request pool, and the collections keep what earlier points appended).
