@@ -243,12 +243,18 @@ __device__ __forceinline__ float mfma_distance(float am, float aq, float fm, flo
 // (q_hi*c_lo, s_lo*(+-1)) move a distance by up to a few 1e-5 of that scale,
 // which is a few 1e-5 relative to every distance above the bar; the pairs
 // below it (a query's near codes, the ones a top-k decides on) get the
-// dropped terms added (refine_block), which makes them the full pass set's
+// dropped terms added (refine_flagged), which makes them the full pass set's
 // value (only q_lo*c_lo, ~2^-22 relative, stays out).
 constexpr float kRefineFrac = 0.5f;
-// Corrections run kRefinePairs pairs at a time, kRefineLanes lanes each.
-constexpr int kRefineLanes = 8;
-constexpr int kRefinePairs = 64 / kRefineLanes;
+// The dropped terms are summed as 32-bit fixed-point integers, so the sums are
+// the same in any order and lane grouping: q_hi * c_lo at 2^13 (|q_hi| <=
+// 2^10 |q|, |c_lo| <= 0.25, so the sum of magnitudes is below 2^8 sqrt(D) |q|
+// <= 2^14.5 |q| at D = 8192: no overflow below |q| = 11, and prepared rows
+// have |q| = 1), each fp32 product rounded once (2^-14, ~1e-9 of a near pair's
+// sum); s_lo (fp16, multiples of 2^-24) at 2^22 (sum of magnitudes <= 2^5.5
+// |s| at D = 8192).
+constexpr int kRefineShiftM = 13;
+constexpr int kRefineShiftQ = 22;
 
 __device__ __forceinline__ bool refine_flag(float d, float qns, float xns) {
   return d < kRefineFrac * (qns + xns);
@@ -257,70 +263,73 @@ __device__ __forceinline__ bool refine_flag(float d, float qns, float xns) {
 // The terms the coarse passes drop for one (query, 4-bit code) pair, in the
 // MFMA sums' units: dm = sum_i q_hi[i] * c_lo[idx_i], dq = sum_i s_lo[i] *
 // (+-1)_i, from the query's correction row (tq_mfma_dev.h) and the code's
-// nibbles, by the kRefineLanes lanes of one eighth of a wavefront: lane
-// `sub` takes the 8-coordinate groups sub, sub + 8, ...; the partial sums
-// meet in a butterfly whose grouping is the same in every lane, so the
-// eight lanes return the same bits, and so does any other eighth given the
-// same pair. l_clo:
-// the space's 8 scaled centroid remainders c_lo in LDS. QJL = false: the
-// symmetric MSE form (no s term).
+// nibbles, by the L lanes of one L-lane group of a wavefront (L a power of
+// two): lane `sub` takes the 8-coordinate groups sub, sub + L, ...; the
+// partial sums are integers (kRefineShiftM / kRefineShiftQ), so every L and
+// every lane gives the same bits. l_clo: the space's 8 scaled centroid
+// remainders c_lo in LDS. QJL = false: the symmetric MSE form (no s term).
 template <bool QJL>
 __device__ __forceinline__ void refine_delta(const SpaceDev &s, const float *l_clo,
-                                          const _Float16 *__restrict__ crow,
-                                          const uint8_t *__restrict__ code, int sub, float &dm,
-                                          float &dq) {
-  float am = 0.0f, aq = 0.0f;
+                                             const _Float16 *__restrict__ crow,
+                                             const uint8_t *__restrict__ code, int sub, int L,
+                                             float &dm, float &dq) {
+  int am = 0, aq = 0;
   const int ng = s.D >> 3;
   // Eight groups per step, all loads issued before the first use (a group
   // past the end loads group `sub` again and is not added).
-  for (int g0 = sub; g0 < ng; g0 += 8 * kRefineLanes) {
+  for (int g0 = sub; g0 < ng; g0 += 8 * L) {
     uint32_t w[8];
     h8 qh[8], sl[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const int g = g0 + kRefineLanes * t < ng ? g0 + kRefineLanes * t : sub;
+      const int g = g0 + L * t < ng ? g0 + L * t : sub;
       w[t] = *reinterpret_cast<const uint32_t *>(code + 4 * g);
       qh[t] = *reinterpret_cast<const h8 *>(crow + g * kCorrHalves);
       if constexpr (QJL) sl[t] = *reinterpret_cast<const h8 *>(crow + g * kCorrHalves + 8);
     }
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      if (g0 + kRefineLanes * t >= ng) break;
+      if (g0 + L * t >= ng) break;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const uint32_t u = (w[t] >> (4 * split_perm(true, e))) & 15u;  // element e's nibble
-        am = __builtin_fmaf((float)qh[t][e], l_clo[u >> 1], am);
-        if constexpr (QJL) aq += (u & 1u) ? (float)sl[t][e] : -(float)sl[t][e];
+        am += __float2int_rn(__builtin_ldexpf((float)qh[t][e] * l_clo[u >> 1], kRefineShiftM));
+        if constexpr (QJL) {
+          const int v = __float2int_rn(__builtin_ldexpf((float)sl[t][e], kRefineShiftQ));
+          aq += (u & 1u) ? v : -v;
+        }
       }
     }
   }
-#pragma unroll
-  for (int o = kRefineLanes / 2; o > 0; o >>= 1) {  // lane ^ o stays within the eighth
+  for (int o = L >> 1; o > 0; o >>= 1) {  // lane ^ o stays within the group
     am += __shfl_xor(am, o);
     if constexpr (QJL) aq += __shfl_xor(aq, o);
   }
-  dm = am;
-  dq = aq;
+  dm = __builtin_ldexpf((float)am, -kRefineShiftM);
+  dq = __builtin_ldexpf((float)aq, -kRefineShiftQ);
 }
 
 // A wavefront's flagged pairs (bit b of `flag` in lane l: pair pos(b, l) =
 // (query row, code column), b = 16 * block + element) get the dropped terms
-// computed by refine_delta, up to kRefinePairs pairs at a time (one per
-// eighth of the wavefront, at most one per lane), taken in lane then bit
-// order; the
-// owning lane then calls finish(b, dm, dq) (b differs between lanes). `corr`
-// is the batch's correction rows. Every lane of the wavefront must call it.
+// computed by refine_delta, up to eight pairs per round taken in lane then
+// bit order (at most one per lane), the wavefront split into as many equal
+// lane groups as the round has pairs (rounded up to 1, 2, 4 or 8): a lone
+// pair gets all 64 lanes. The owning lane then calls finish(b, dm, dq) (b
+// differs between lanes). `corr` is the batch's correction rows. Every lane
+// of the wavefront must call it.
 template <bool QJL, typename Pos, typename Finish>
 __device__ __forceinline__ void refine_flagged(const SpaceDev &s, const float *l_clo,
                                                const _Float16 *__restrict__ corr,
                                                const uint8_t *__restrict__ codes, uint64_t flag,
                                                int lane, Pos &&pos, Finish &&finish) {
+  constexpr int kMaxPairs = 8;
   uint64_t any = __ballot(flag != 0u);
   while (any) {
-    int src[kRefinePairs], bit[kRefinePairs];
-    bool ok[kRefinePairs];
+    int src[kMaxPairs], bit[kMaxPairs];
+    bool ok[kMaxPairs];
+    int np = 0;
 #pragma unroll
-    for (int p = 0; p < kRefinePairs; ++p) {
+    for (int p = 0; p < kMaxPairs; ++p) {
       ok[p] = any != 0;
       if (ok[p]) {
         src[p] = __builtin_ctzll(any);
@@ -329,29 +338,33 @@ __device__ __forceinline__ void refine_flagged(const SpaceDev &s, const float *l
         const uint32_t hi =
             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(flag >> 32), src[p]);
         bit[p] = __builtin_ctzll(((uint64_t)hi << 32) | lo);
-      } else {  // an idle eighth scores pair 0 again and keeps nothing
+        np = p + 1;
+      } else {  // an idle group scores pair 0 again and keeps nothing
         src[p] = src[0];
         bit[p] = bit[0];
       }
     }
+    const int groups = np <= 1 ? 1 : np <= 2 ? 2 : np <= 4 ? 4 : 8;  // wavefront-uniform
+    const int L = 64 / groups;
+    const int grp = lane / L;
     long q = 0, j = 0;
 #pragma unroll
-    for (int p = 0; p < kRefinePairs; ++p) {
+    for (int p = 0; p < kMaxPairs; ++p) {
       long qp, jp;
       pos(bit[p], src[p], qp, jp);
-      if (lane / kRefineLanes == p) {
+      if (grp == p) {
         q = qp;
         j = jp;
       }
     }
     float dm, dq;
     refine_delta<QJL>(s, l_clo, corr + q * (long)(s.D / 8) * kCorrHalves, codes + j * (long)s.cs,
-                      lane % kRefineLanes, dm, dq);
+                      lane % L, L, dm, dq);
     int mb = -1;
     float mdm = 0.0f, mdq = 0.0f;
 #pragma unroll
-    for (int p = 0; p < kRefinePairs; ++p) {
-      const int l0 = kRefineLanes * p;
+    for (int p = 0; p < kMaxPairs; ++p) {
+      const int l0 = (L * p) & 63;  // group p's first lane (p < groups whenever ok[p])
       const float pm = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dm), l0));
       const float pq = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dq), l0));
       if (ok[p] && lane == src[p]) {

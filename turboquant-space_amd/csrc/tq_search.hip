@@ -159,52 +159,60 @@ __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, lo
     xnv[c] = xn;
     xns[c] = xn * xn;
   }
-  const int h = QR == 1 ? (lane >> 5) : 0;
-  const long cbase = jbase + 32 * h;
-#pragma unroll
-  for (int r = 0; r < QR; ++r) {
-    // k_mfma's epilogue: the same distances, the coarse form's flagged pairs
-    // (bit 16 c + v) corrected the same way.
+  // k_mfma's epilogue: the same distances, and the coarse form's flagged pairs
+  // (bit 32 r + 16 c + v) corrected the same way, here by adding the dropped
+  // terms to the sums before the distances are formed (the same fp32 sum
+  // k_mfma forms), all of the wavefront's blocks in one pass.
+  if constexpr (CO) {
     uint64_t flag = 0;
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const bool jin = jbase + 32 * c + lr < n;
+    for (int r = 0; r < QR; ++r)
 #pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int row = 8 * (v >> 2) + 4 * kg + (v & 3);
-        const int qrow = 32 * r + row;
-        const float d = mfma_distance<true>(acc_m[r][c][v], acc_q[r][c][v], fm[c], fq[c], xnv[c],
-                                            xns[c], l_qn[qrow][0], l_qn[qrow][1]);
-        t_lds[wave][row][32 * c + lr] = d;
-        if constexpr (CO)
+      for (int c = 0; c < 2; ++c) {
+        const bool jin = jbase + 32 * c + lr < n;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int qrow = 32 * r + 8 * (v >> 2) + 4 * kg + (v & 3);
+          const float d = mfma_distance<true>(acc_m[r][c][v], acc_q[r][c][v], fm[c], fq[c],
+                                              xnv[c], xns[c], l_qn[qrow][0], l_qn[qrow][1]);
           if (jin && q0 + qrow < m && refine_flag(d, l_qn[qrow][0], xns[c]))
-            flag |= 1ull << (16 * c + v);
+            flag |= 1ull << (32 * r + 16 * c + v);
+        }
       }
-    }
-    if constexpr (CO)
-      refine_flagged<true>(
-          s, l_clo, job.planes + split_planes_halves(s.D, m, Sh::NPL), job.codes, flag, lane,
-          [&](int b, int src, long &q, long &j) {
-            const int v = b & 15;
-            q = q0 + 32 * r + 8 * (v >> 2) + 4 * (src >> 5) + (v & 3);
-            j = jbase + 32 * (b >> 4) + (src & 31);
-          },
-          [&](int b, float dm, float dq) {
-            float am = 0.0f, aq = 0.0f;
+    refine_flagged<true>(
+        s, l_clo, job.planes + split_planes_halves(s.D, m, Sh::NPL), job.codes, flag, lane,
+        [&](int b, int src, long &q, long &j) {
+          const int v = b & 15;
+          q = q0 + 32 * (b >> 5) + 8 * (v >> 2) + 4 * (src >> 5) + (v & 3);
+          j = jbase + 32 * ((b >> 4) & 1) + (src & 31);
+        },
+        [&](int b, float dm, float dq) {
+#pragma unroll
+          for (int r = 0; r < QR; ++r)
 #pragma unroll
             for (int c = 0; c < 2; ++c)
 #pragma unroll
               for (int v = 0; v < 16; ++v)
-                if (b == 16 * c + v) {
-                  am = acc_m[r][c][v];
-                  aq = acc_q[r][c][v];
+                if (b == 32 * r + 16 * c + v) {
+                  acc_m[r][c][v] = acc_m[r][c][v] + dm;
+                  acc_q[r][c][v] = acc_q[r][c][v] + dq;
                 }
-            const int v = b & 15, c = b >> 4;
-            const int row = 8 * (v >> 2) + 4 * kg + (v & 3);
-            t_lds[wave][row][32 * c + lr] = mfma_distance<true>(
-                am + dm, aq + dq, c ? fm[1] : fm[0], c ? fq[1] : fq[0], c ? xnv[1] : xnv[0],
-                c ? xns[1] : xns[0], l_qn[32 * r + row][0], l_qn[32 * r + row][1]);
-          });
+        });
+  }
+  const int h = QR == 1 ? (lane >> 5) : 0;
+  const long cbase = jbase + 32 * h;
+#pragma unroll
+  for (int r = 0; r < QR; ++r) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int row = 8 * (v >> 2) + 4 * kg + (v & 3);
+        const int qrow = 32 * r + row;
+        t_lds[wave][row][32 * c + lr] = mfma_distance<true>(acc_m[r][c][v], acc_q[r][c][v], fm[c],
+                                                            fq[c], xnv[c], xns[c], l_qn[qrow][0],
+                                                            l_qn[qrow][1]);
+      }
     __syncthreads();
     const bool mine = QR == 1 || (lane >> 5) == r;
     const float *rowp = &t_lds[wave][lane & 31][32 * h];
