@@ -273,7 +273,10 @@ __device__ __forceinline__ void refine_delta(const SpaceDev &s, const float *l_c
                                              const _Float16 *__restrict__ crow,
                                              const uint8_t *__restrict__ code, int sub, int L,
                                              float &dm, float &dq) {
-  int am = 0, aq = 0;
+  // Unsigned sums: wrap-around instead of undefined behaviour should a row
+  // be far outside the bound (codes with garbage trailers in the symmetric
+  // form); in range they are the two's-complement sums.
+  uint32_t am = 0u, aq = 0u;
   const int ng = s.D >> 3;
   // Eight groups per step, all loads issued before the first use (a group
   // past the end loads group `sub` again and is not added).
@@ -293,20 +296,22 @@ __device__ __forceinline__ void refine_delta(const SpaceDev &s, const float *l_c
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const uint32_t u = (w[t] >> (4 * split_perm(true, e))) & 15u;  // element e's nibble
-        am += __float2int_rn(__builtin_ldexpf((float)qh[t][e] * l_clo[u >> 1], kRefineShiftM));
+        am += (uint32_t)__float2int_rn(
+            __builtin_ldexpf((float)qh[t][e] * l_clo[u >> 1], kRefineShiftM));
         if constexpr (QJL) {
-          const int v = __float2int_rn(__builtin_ldexpf((float)sl[t][e], kRefineShiftQ));
-          aq += (u & 1u) ? v : -v;
+          const uint32_t v =
+              (uint32_t)__float2int_rn(__builtin_ldexpf((float)sl[t][e], kRefineShiftQ));
+          aq += (u & 1u) ? v : 0u - v;
         }
       }
     }
   }
   for (int o = L >> 1; o > 0; o >>= 1) {  // lane ^ o stays within the group
-    am += __shfl_xor(am, o);
-    if constexpr (QJL) aq += __shfl_xor(aq, o);
+    am += (uint32_t)__shfl_xor((int)am, o);
+    if constexpr (QJL) aq += (uint32_t)__shfl_xor((int)aq, o);
   }
-  dm = __builtin_ldexpf((float)am, -kRefineShiftM);
-  dq = __builtin_ldexpf((float)aq, -kRefineShiftQ);
+  dm = __builtin_ldexpf((float)(int)am, -kRefineShiftM);
+  dq = __builtin_ldexpf((float)(int)aq, -kRefineShiftQ);
 }
 
 // A wavefront's flagged pairs (bit b of `flag` in lane l: pair pos(b, l) =
