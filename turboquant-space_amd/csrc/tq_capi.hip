@@ -249,8 +249,9 @@ int tq_space_create(uint64_t dim, int bits, uint64_t rot_seed, uint64_t qjl_seed
   d.inv_sqrt_D = 1.0f / std::sqrt(static_cast<float>(D));
   // Centroids times 2^8 stay within fp16's normal range together with their
   // fp16 remainders (|c| < 4 at every bit width); the coarse form of 4-bit
-  // spaces takes its single-term scale (coarse_centroid_scale).
-  d.cscale = tq::mfma_coarse(d) ? coarse_centroid_scale(s->cent) : 256.0f;
+  // spaces takes its single-term scale (coarse_centroid_scale: the 8
+  // centroids of 3 MSE bits fit one; byte codes' 128 do not and keep 2^8).
+  d.cscale = tq::mfma_coarse(d) && d.nib ? coarse_centroid_scale(s->cent) : 256.0f;
   d.mse_unscale = (float)(1.0 / (1024.0 * (double)d.cscale));
   d.cent = s->d_tables;
   d.bnd = s->d_tables + L;

@@ -35,10 +35,18 @@ struct SpaceDev {
   const float *qjl;   // [D]   +-1 QJL signs (qjl_seed)
 };
 
-// MFMA forms: 4-bit spaces (3 MSE bits + the QJL bit) run the coarse pass
-// set (q_hi*c_hi, q_lo*c_hi, s_hi*(+-1)) with the near pairs recomputed in
-// fp32 (refine_pair, tq_mfma_tile.h); other bit widths run every lo pass.
-__host__ __device__ inline bool mfma_coarse(const SpaceDev &s) { return s.nib && s.bits == 4; }
+// MFMA forms: 4-bit spaces (3 MSE bits + the QJL bit) and byte-code spaces
+// of padded dim >= 1024 run the coarse pass set (q_hi*c_hi, q_lo*c_hi,
+// s_hi*(+-1)) with the near pairs' dropped terms added in fixed point
+// (refine_flagged, tq_mfma_tile.h); 2-3-bit spaces and smaller dims run every
+// lo pass. Below 1024 the passes saved cost less than the coarse epilogue's
+// flags and corrections on clustered collections (a query's cluster-mates
+// are all near pairs), and at padded dim 128 the dropped q_hi*c_lo term of
+// byte codes moves far pairs by up to ~9e-5 relative, too close to the 1e-4
+// bar.
+__host__ __device__ inline bool mfma_coarse(const SpaceDev &s) {
+  return s.D >= 1024 && (s.nib ? s.bits == 4 : true);
+}
 
 // fp32 trailer field at any byte alignment (4-bit codes at D = 4 are 14 bytes
 // long, so their trailers are only 2-byte aligned).

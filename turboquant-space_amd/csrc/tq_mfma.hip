@@ -19,13 +19,15 @@
 // fp16's normal range; the epilogue removes the scales. With every lo pass
 // in, the error against the fp64 restatement of the estimator is a few 1e-6
 // relative to the distance on each query's nearest codes.
-// 4-bit spaces run a coarse pass set instead (mfma_coarse): q_hi*c_hi,
-// q_lo*c_hi and s_hi*(+-1), three passes instead of five. The dropped terms
-// move a distance by up to a few 1e-5 of (|q|^2 + |x|^2); so every pair whose
-// coarse distance is below half of that scale (a query's near codes, the
-// distances a top-k decides on) is recomputed in fp32 in the epilogue
-// (refine_block, tq_mfma_tile.h), and the pairs above it stay within a few
-// 1e-5 relative. Both parts are functions of the (query, code) pair alone.
+// 4-bit and byte-code spaces at padded dim >= 1024 run a coarse pass set
+// instead (mfma_coarse): q_hi*c_hi, q_lo*c_hi and s_hi*(+-1), three passes
+// instead of five (4-bit) or four (bytes, which never take s_lo). The
+// dropped terms move a distance by up to a few 1e-5 of (|q|^2 + |x|^2); so
+// every pair whose coarse distance is below half of that scale (a query's
+// near codes, the distances a top-k decides on) gets them added in the
+// epilogue, summed in fixed point (refine_flagged, tq_mfma_tile.h), and the
+// pairs above it stay within a few 1e-5 relative. Both parts are functions
+// of the (query, code) pair alone.
 //
 // Tiling: a workgroup (4 wavefronts) owns 64 (or 32) queries x 256 codes;
 // each wavefront owns the queries x 64 codes as 2 x 2 (or 1 x 2) blocks of
@@ -118,7 +120,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   // entry k at 32 k + c), lane l reading copy l % 32 (decode8<true>).
   __shared__ uint32_t l_tab[NIB ? 1 : 128 * 32];
   __shared__ float l_qn[QT][2];  // q_norm_sq, q_norm of the tile's queries
-  __shared__ float l_clo[8];     // coarse form: the centroid remainders, for refine_block
+  // coarse form: the centroid remainders (8 or 128), for refine_flagged
+  __shared__ float l_clo[CO ? (NIB ? 8 : 128) : 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     l_qn[tid][1] = pq[2 * s.D + 1];
   }
   if constexpr (CO)
-    if (tid < 8) l_clo[tid] = cent_lo(s, tid);  // 4-bit: 8 centroids
+    if (tid < (NIB ? 8 : 128)) l_clo[tid] = cent_lo(s, tid);
   if constexpr (!NIB) {
     for (int i = tid; i < 128 * 32; i += blockDim.x) {
       _Float16 ch, cl;
@@ -185,28 +188,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     xnv[c] = xn;
     xns[c] = xn * xn;
   }
+  // Tiles inside both bounds (all but the last of each dimension) store
+  // without per-element checks; each lane's element rows are 4 kg plus a
+  // compile-time offset rr, so every store address is the lane's base plus a
+  // wavefront-uniform rr * n.
   uint64_t flag = 0;
+  float *const obase = out + (q0 + 4 * kg) * n + j0 + wave * 64 + lr;
+  auto epilogue = [&](auto full) {
+    constexpr bool FULL = decltype(full)::value;
 #pragma unroll
-  for (int r = 0; r < RB; ++r)
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int r = 0; r < RB; ++r)
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const int row = 32 * r + 8 * (v >> 2) + 4 * kg + (v & 3);
-        const long q = q0 + row;
-        float aq = 0.0f;
-        if constexpr (QJL) aq = acc_q[r][c][v];
-        const float d = mfma_distance<QJL>(acc_m[r][c][v], aq, fm[c], fq[c], xnv[c], xns[c],
-                                           l_qn[row][0], l_qn[row][1]);
-        if (jin[c] && q < m) {
-          out[q * n + j0 + wave * 64 + 32 * c + lr] = d;
-          if constexpr (CO)
-            if (refine_flag(d, l_qn[row][0], xns[c])) flag |= 1ull << (16 * (2 * r + c) + v);
+        const int rr = 32 * r + 8 * (v >> 2) + (v & 3);
+        const int row = rr + 4 * kg;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          float aq = 0.0f;
+          if constexpr (QJL) aq = acc_q[r][c][v];
+          const float d = mfma_distance<QJL>(acc_m[r][c][v], aq, fm[c], fq[c], xnv[c], xns[c],
+                                             l_qn[row][0], l_qn[row][1]);
+          if constexpr (FULL) {
+            obase[(long)rr * n + 32 * c] = d;
+            if constexpr (CO)
+              flag |= (uint64_t)refine_flag(d, l_qn[row][0], xns[c]) << (16 * (2 * r + c) + v);
+          } else if (jin[c] && q0 + row < m) {
+            obase[(long)rr * n + 32 * c] = d;
+            if constexpr (CO)
+              if (refine_flag(d, l_qn[row][0], xns[c])) flag |= 1ull << (16 * (2 * r + c) + v);
+          }
         }
       }
+  };
+  if (q0 + QT <= m && j0 + kTN <= n)
+    epilogue(std::true_type{});
+  else
+    epilogue(std::false_type{});
   if constexpr (CO) {
-    refine_flagged<QJL>(
-        s, l_clo, planes + split_planes_halves(s.D, m, Sh::NPL), codes, flag, lane,
+    refine_flagged<NIB, QJL>(
+        s, [&](uint32_t k) { return l_clo[k]; }, planes + split_planes_halves(s.D, m, Sh::NPL), codes, flag, lane,
         [&](int b, int src, long &q, long &j) {
           const int v = b & 15;
           q = q0 + 32 * (b >> 5) + 8 * (v >> 2) + 4 * (src >> 5) + (v & 3);
@@ -260,12 +280,15 @@ namespace {
 template <int QT, bool QJL>
 void launch_tile(const SpaceDev &s, dim3 grid, const _Float16 *planes, const float *prep, long m,
                  const uint8_t *codes, long n, float *out, hipStream_t st) {
-  if (mfma_coarse(s))
+  if (mfma_coarse(s) && s.nib)
     hipLaunchKernelGGL((k_mfma<true, QT, QJL, true>), grid, dim3(256), 0, st, s, planes, prep, m,
                        codes, n, out);
   else if (s.nib)
     hipLaunchKernelGGL((k_mfma<true, QT, QJL, false>), grid, dim3(256), 0, st, s, planes, prep, m,
                        codes, n, out);
+  else if (mfma_coarse(s))
+    hipLaunchKernelGGL((k_mfma<false, QT, QJL, true>), grid, dim3(256), 0, st, s, planes, prep,
+                       m, codes, n, out);
   else
     hipLaunchKernelGGL((k_mfma<false, QT, QJL, false>), grid, dim3(256), 0, st, s, planes, prep,
                        m, codes, n, out);

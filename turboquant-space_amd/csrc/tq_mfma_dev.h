@@ -40,7 +40,7 @@ __host__ __device__ inline long split_planes_halves(int D, long m, int npl) {
 // The coarse form (mfma_coarse) adds, after the planes of the padded rows,
 // one correction row per query: for each group g of 8 coordinates, 16 halves
 // = q_hi of the group's coordinates, then s_lo, both in the planes' element
-// order (perm below). refine_block reads it for the terms the coarse passes
+// order (perm below). refine_flagged reads it for the terms the coarse passes
 // drop (q_hi * c_lo, s_lo * (+-1)).
 constexpr int kCorrHalves = 16;  // halves per 8-coordinate group
 __host__ __device__ inline long corr_halves(int D, long m) {

@@ -17,11 +17,10 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // The same at 4-byte alignment (code rows are 4-byte aligned).
 typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
 
-// CO: the coarse pass set of 4-bit spaces (mfma_coarse): no s_lo plane and
-// no q_hi*c_lo pass; the caller recomputes the near pairs (refine_block).
+// CO: the coarse pass set (mfma_coarse): no s_lo plane and no q_hi*c_lo
+// pass; the caller corrects the near pairs (refine_flagged).
 template <bool NIB, int QT, bool QJL, bool CO = false>
 struct MfmaShape {
-  static_assert(NIB || !CO, "the coarse pass set is for nibble codes");
   static constexpr int RB = QT / 32;                          // 32-row blocks per wavefront
   static constexpr int NPL = split_planes(NIB, QJL, CO);      // query planes read
   static constexpr int NB = NPL * RB * 2;                     // 1 KB A blocks per chunk
@@ -29,9 +28,10 @@ struct MfmaShape {
   static constexpr int NC = NIB ? 1 : 2;                      // 16-byte code loads per thread
   static constexpr int A_BYTES = NB * 1024;
   static constexpr int STAGE = A_BYTES + 4 * NC * 1024;
-  // Chunks whose loads are in the register ring: three where the registers
-  // allow it, two for the 64-query tiles with four QJL-pass planes.
-  static constexpr int RING = (QT == 64 && QJL && !CO) ? 2 : 3;
+  // Chunks whose loads are in the register ring: three where a chunk takes
+  // at most four 16-byte loads per thread, else two (the 64-query QJL tiles
+  // of 4-bit codes with every pass, and of byte codes).
+  static constexpr int RING = ALOADS + NC <= 4 ? 3 : 2;
 };
 
 // acc_m / acc_q of queries [q0, q0 + QT) x codes [j0, j0 + 256) of one
@@ -246,13 +246,14 @@ __device__ __forceinline__ float mfma_distance(float am, float aq, float fm, flo
 // dropped terms added (refine_flagged), which makes them the full pass set's
 // value (only q_lo*c_lo, ~2^-22 relative, stays out).
 constexpr float kRefineFrac = 0.5f;
-// The dropped terms are summed as 32-bit fixed-point integers, so the sums are
-// the same in any order and lane grouping: q_hi * c_lo at 2^13 (|q_hi| <=
-// 2^10 |q|, |c_lo| <= 0.25, so the sum of magnitudes is below 2^8 sqrt(D) |q|
-// <= 2^14.5 |q| at D = 8192: no overflow below |q| = 11, and prepared rows
-// have |q| = 1), each fp32 product rounded once (2^-14, ~1e-9 of a near pair's
-// sum); s_lo (fp16, multiples of 2^-24) at 2^22 (sum of magnitudes <= 2^5.5
-// |s| at D = 8192).
+// The dropped terms of each 8-coordinate group are summed in fp32 by one
+// lane in a fixed order, and the group sums as 32-bit fixed-point integers,
+// so the totals are the same in any lane grouping: q_hi * c_lo at 2^13
+// (|q_hi| <= 2^10 |q|, |c_lo| <= 0.25, so the sum of magnitudes is below
+// 2^8 sqrt(D) |q| <= 2^14.5 |q| at D = 8192: no overflow below |q| = 11, and
+// prepared rows have |q| = 1), each group sum rounded once (2^-14, ~1e-9 of a
+// near pair's sum); s_lo (fp16, multiples of 2^-24) at 2^22 (sum of
+// magnitudes <= 2^5.5 |s| at D = 8192).
 constexpr int kRefineShiftM = 13;
 constexpr int kRefineShiftQ = 22;
 
@@ -260,19 +261,22 @@ __device__ __forceinline__ bool refine_flag(float d, float qns, float xns) {
   return d < kRefineFrac * (qns + xns);
 }
 
-// The terms the coarse passes drop for one (query, 4-bit code) pair, in the
-// MFMA sums' units: dm = sum_i q_hi[i] * c_lo[idx_i], dq = sum_i s_lo[i] *
-// (+-1)_i, from the query's correction row (tq_mfma_dev.h) and the code's
-// nibbles, by the L lanes of one L-lane group of a wavefront (L a power of
-// two): lane `sub` takes the 8-coordinate groups sub, sub + L, ...; the
-// partial sums are integers (kRefineShiftM / kRefineShiftQ), so every L and
-// every lane gives the same bits. l_clo: the space's 8 scaled centroid
-// remainders c_lo in LDS. QJL = false: the symmetric MSE form (no s term).
-template <bool QJL>
-__device__ __forceinline__ void refine_delta(const SpaceDev &s, const float *l_clo,
+// The terms the coarse passes drop for one (query, code) pair, in the MFMA
+// sums' units: dm = sum_i q_hi[i] * c_lo[idx_i], and for 4-bit codes with
+// QJL dq = sum_i s_lo[i] * (+-1)_i (byte codes keep no s_lo term in either
+// pass set), from the query's correction row (tq_mfma_dev.h) and the code's
+// units, by the L lanes of one L-lane group of a wavefront (L a power of
+// two): lane `sub` takes the 8-coordinate groups sub, sub + L, ...; each
+// group's fp32 sum (elements 0..7 in order) becomes an integer
+// (kRefineShiftM / kRefineShiftQ) before the lanes' sums meet, so every L
+// and every lane gives the same bits. clo(k): the scaled centroid remainder c_lo
+// of index k (from LDS). QJL = false: the symmetric MSE form (no s term).
+template <bool NIB, bool QJL, typename Clo>
+__device__ __forceinline__ void refine_delta(const SpaceDev &s, Clo &&clo,
                                              const _Float16 *__restrict__ crow,
                                              const uint8_t *__restrict__ code, int sub, int L,
                                              float &dm, float &dq) {
+  constexpr bool SQ = QJL && NIB;  // an s_lo term to add
   // Unsigned sums: wrap-around instead of undefined behaviour should a row
   // be far outside the bound (codes with garbage trailers in the symmetric
   // form); in range they are the two's-complement sums.
@@ -281,37 +285,42 @@ __device__ __forceinline__ void refine_delta(const SpaceDev &s, const float *l_c
   // Eight groups per step, all loads issued before the first use (a group
   // past the end loads group `sub` again and is not added).
   for (int g0 = sub; g0 < ng; g0 += 8 * L) {
-    uint32_t w[8];
+    uint2 w[8];
     h8 qh[8], sl[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int g = g0 + L * t < ng ? g0 + L * t : sub;
-      w[t] = *reinterpret_cast<const uint32_t *>(code + 4 * g);
+      if constexpr (NIB) {
+        w[t].x = *reinterpret_cast<const uint32_t *>(code + 4 * g);
+        w[t].y = 0u;
+      } else {
+        w[t] = *reinterpret_cast<const uint2 *>(code + 8 * g);
+      }
       qh[t] = *reinterpret_cast<const h8 *>(crow + g * kCorrHalves);
-      if constexpr (QJL) sl[t] = *reinterpret_cast<const h8 *>(crow + g * kCorrHalves + 8);
+      if constexpr (SQ) sl[t] = *reinterpret_cast<const h8 *>(crow + g * kCorrHalves + 8);
     }
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       if (g0 + L * t >= ng) break;
+      float pm = 0.0f, pq = 0.0f;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        const uint32_t u = (w[t] >> (4 * split_perm(true, e))) & 15u;  // element e's nibble
-        am += (uint32_t)__float2int_rn(
-            __builtin_ldexpf((float)qh[t][e] * l_clo[u >> 1], kRefineShiftM));
-        if constexpr (QJL) {
-          const uint32_t v =
-              (uint32_t)__float2int_rn(__builtin_ldexpf((float)sl[t][e], kRefineShiftQ));
-          aq += (u & 1u) ? v : 0u - v;
-        }
+        // element e's unit: nibble perm(e) of the word, or byte e of the pair
+        const uint32_t u = NIB ? (w[t].x >> (4 * split_perm(true, e))) & 15u
+                               : ((e < 4 ? w[t].x : w[t].y) >> (8 * (e & 3))) & 0xFFu;
+        pm = __builtin_fmaf((float)qh[t][e], clo(u >> 1), pm);
+        if constexpr (SQ) pq += (u & 1u) ? (float)sl[t][e] : -(float)sl[t][e];
       }
+      am += (uint32_t)__float2int_rn(__builtin_ldexpf(pm, kRefineShiftM));
+      if constexpr (SQ) aq += (uint32_t)__float2int_rn(__builtin_ldexpf(pq, kRefineShiftQ));
     }
   }
   for (int o = L >> 1; o > 0; o >>= 1) {  // lane ^ o stays within the group
     am += (uint32_t)__shfl_xor((int)am, o);
-    if constexpr (QJL) aq += (uint32_t)__shfl_xor((int)aq, o);
+    if constexpr (SQ) aq += (uint32_t)__shfl_xor((int)aq, o);
   }
   dm = __builtin_ldexpf((float)(int)am, -kRefineShiftM);
-  dq = __builtin_ldexpf((float)(int)aq, -kRefineShiftQ);
+  dq = SQ ? __builtin_ldexpf((float)(int)aq, -kRefineShiftQ) : 0.0f;
 }
 
 // A wavefront's flagged pairs (bit b of `flag` in lane l: pair pos(b, l) =
@@ -322,8 +331,8 @@ __device__ __forceinline__ void refine_delta(const SpaceDev &s, const float *l_c
 // pair gets all 64 lanes. The owning lane then calls finish(b, dm, dq) (b
 // differs between lanes). `corr` is the batch's correction rows. Every lane
 // of the wavefront must call it.
-template <bool QJL, typename Pos, typename Finish>
-__device__ __forceinline__ void refine_flagged(const SpaceDev &s, const float *l_clo,
+template <bool NIB, bool QJL, typename Clo, typename Pos, typename Finish>
+__device__ __forceinline__ void refine_flagged(const SpaceDev &s, Clo &&clo,
                                                const _Float16 *__restrict__ corr,
                                                const uint8_t *__restrict__ codes, uint64_t flag,
                                                int lane, Pos &&pos, Finish &&finish) {
@@ -363,8 +372,8 @@ __device__ __forceinline__ void refine_flagged(const SpaceDev &s, const float *l
       }
     }
     float dm, dq;
-    refine_delta<QJL>(s, l_clo, corr + q * (long)(s.D / 8) * kCorrHalves, codes + j * (long)s.cs,
-                      lane % L, L, dm, dq);
+    refine_delta<NIB, QJL>(s, clo, corr + q * (long)(s.D / 8) * kCorrHalves,
+                           codes + j * (long)s.cs, lane % L, L, dm, dq);
     int mb = -1;
     float mdm = 0.0f, mdq = 0.0f;
 #pragma unroll
@@ -386,12 +395,13 @@ __device__ __forceinline__ void refine_flagged(const SpaceDev &s, const float *l
   }
 }
 
-// The scaled centroid remainders of a 4-bit space (c * cscale - c_hi, the
-// part of each centroid the coarse passes leave out), for l_clo.
+// The scaled centroid remainder of index k (c * cscale - c_hi, the part of
+// each centroid the coarse passes leave out; 0 past the L centroids), for
+// l_clo.
 __device__ __forceinline__ float cent_lo(const SpaceDev &s, int k) {
   _Float16 ch, cl;
   cent_h(s, k, ch, cl);
-  return s.cent[k] * s.cscale - (float)ch;
+  return k < s.L ? s.cent[k] * s.cscale - (float)ch : 0.0f;
 }
 
 }  // namespace mfma

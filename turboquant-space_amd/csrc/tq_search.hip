@@ -179,8 +179,8 @@ __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, lo
             flag |= 1ull << (32 * r + 16 * c + v);
         }
       }
-    refine_flagged<true>(
-        s, l_clo, job.planes + split_planes_halves(s.D, m, Sh::NPL), job.codes, flag, lane,
+    refine_flagged<NIB, true>(
+        s, [&](uint32_t k) { return l_clo[k]; }, job.planes + split_planes_halves(s.D, m, Sh::NPL), job.codes, flag, lane,
         [&](int b, int src, long &q, long &j) {
           const int v = b & 15;
           q = q0 + 32 * (b >> 5) + 8 * (v >> 2) + 4 * (src >> 5) + (v & 3);
@@ -277,7 +277,7 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
   __shared__ __attribute__((aligned(16))) uint8_t lds[Sh::LDS];
   __shared__ uint32_t l_tab[128];
   __shared__ float l_qn[QT][2];
-  __shared__ float l_clo[8];  // 4-bit jobs: the centroid remainders, for refine_block
+  __shared__ float l_clo[128];  // coarse jobs: the centroid remainders, for refine_flagged
   __shared__ float l_ld[4][K][64];
   __shared__ uint32_t l_li[4][K][64];
   __shared__ int s_item;
@@ -309,7 +309,7 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
         cent_h(s, i, ch, cl);
         l_tab[i] = h_bits(ch) | (h_bits(cl) << 16);
       }
-    if (coarse && tid < 8) l_clo[tid] = cent_lo(s, tid);
+    if (coarse && tid < (nib ? 8 : 128)) l_clo[tid] = cent_lo(s, tid);
     if (tid < QT) {
       const long q = q0 + tid;
       const float *pq = job.prep + (q < m ? q : 0L) * prep_stride(s.D);
@@ -327,11 +327,14 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
     // orders the tables above before its first use of them.)
     for (int tile = item.t0; tile < item.t1; ++tile) {
       const long j0 = (long)tile * kSN;
-      if (coarse)
+      if (coarse && nib)
         search_tile<true, true, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_ld,
                                        l_li);
       else if (nib)
         search_tile<true, false, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_ld,
+                                        l_li);
+      else if (coarse)
+        search_tile<false, true, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_ld,
                                         l_li);
       else
         search_tile<false, false, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_ld,
