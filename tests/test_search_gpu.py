@@ -118,6 +118,33 @@ def test_mfma_search_equals_topk_of_mfma_matrix(gpu, torch_cuda, dim, bits, form
         plan.close()
 
 
+@pytest.mark.parametrize("dim,bits", [(768, 4), (768, 8)])
+def test_coarse_search_over_chunks_equals_topk_of_mfma_matrix(gpu, torch_cuda, dim, bits):
+    # Coarse-form collections (padded dim 1024) of 64 clusters, so every query
+    # has thousands of near pairs. 16 MB distance blocks split the blocked form
+    # into 2-3 chunks: after the first, a block corrects only the near pairs
+    # that can still beat the running k-th best; the fused form prunes against
+    # its lists. Both must return the top-k of the fully corrected matrix.
+    torch, capi = torch_cuda, gpu
+    n = 150_000
+    c = Coll(torch, capi, dim, bits, n, seed=4000 + bits, dup=500)
+    for m in (40, 64):
+        for form in (capi.TQ_FORM_BLOCKED, capi.TQ_FORM_FUSED):
+            say(f"[gpu] coarse search over chunks dim={dim} bits={bits} m={m} form={form}")
+            Q, dQ = c.queries(torch, m)
+            k = 10
+            od, oi = outputs(torch, m, k)
+            plan = capi.SearchPlan([(c.sp, dQ, m, c.codes, n, od, oi)], k, capi.TQ_MODE_MFMA,
+                                   form=form, block_mb=16)
+            plan.run()
+            torch.cuda.synchronize()
+            want_d, want_i = host_topk(full_matrix(torch, capi, c, dQ, m, capi.TQ_MODE_MFMA), k)
+            np.testing.assert_array_equal(oi[:m].cpu().numpy(), want_i)
+            np.testing.assert_array_equal(od[:m].cpu().numpy().view(np.uint32),
+                                          want_d.view(np.uint32))
+            plan.close()
+
+
 @pytest.mark.parametrize("form", [1, 2])
 @pytest.mark.parametrize("k", [1, 16])
 def test_mfma_search_k_bounds(gpu, torch_cuda, form, k):

@@ -253,6 +253,14 @@ int tq_space_create(uint64_t dim, int bits, uint64_t rot_seed, uint64_t qjl_seed
   // centroids of 3 MSE bits fit one; byte codes' 128 do not and keep 2^8).
   d.cscale = tq::mfma_coarse(d) && d.nib ? coarse_centroid_scale(s->cent) : 256.0f;
   d.mse_unscale = (float)(1.0 / (1024.0 * (double)d.cscale));
+  // The same fp32 product and fp16 rounding as the device's cent_lo; code
+  // units index at most 128 centroids (a byte holds 7 index bits).
+  d.clo_max = 0.0f;
+  if (tq::mfma_coarse(d))
+    for (int k = 0; k < (int)std::min<size_t>(L, 128); ++k) {
+      const float c = s->cent[k] * d.cscale;
+      d.clo_max = std::max(d.clo_max, std::fabs(c - (float)(_Float16)c));
+    }
   d.cent = s->d_tables;
   d.bnd = s->d_tables + L;
   d.rot = s->d_tables + 2 * L;

@@ -29,6 +29,9 @@ struct SpaceDev {
   // epilogue.
   float cscale;
   float mse_unscale;
+  // Coarse form: the largest |c * cscale - fp16(c * cscale)| over the
+  // centroids the code units can index (bounds the near pairs' correction).
+  float clo_max;
   const float *cent;  // [L]   Lloyd-Max centroids (ascending)
   const float *bnd;   // [L-1] decision boundaries (ascending)
   const float *rot;   // [D]   +-1 rotation signs (rot_seed)

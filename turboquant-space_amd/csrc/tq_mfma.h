@@ -18,8 +18,12 @@ hipError_t launch_split_queries(const SpaceDev &s, const float *prep, long m,
 // The MFMA distances [m][n] of queries already split into `planes` (written
 // by launch_split_queries with the same qjl, m <= 65535 * 64), no allocation.
 // qt: queries per workgroup tile, 32 or 64 (0: 32 for m <= 32, else 64);
-// every element is the same whichever tile computes it.
+// every element is the same whichever tile computes it. thr (coarse form
+// only, for a top-k over the block): query q's running k-th best distance at
+// thr[q * thr_stride]; near pairs that provably cannot beat it are left
+// uncorrected (their entries are then only good for that top-k).
 hipError_t launch_mfma_planes(const SpaceDev &s, const _Float16 *planes, const float *prep, long m,
                               const uint8_t *codes, long n, float *out, hipStream_t st,
-                              bool qjl = true, int qt = 0);
+                              bool qjl = true, int qt = 0, const float *thr = nullptr,
+                              long thr_stride = 0);
 }  // namespace tq

@@ -107,11 +107,15 @@ __global__ __launch_bounds__(256) void k_split_queries(SpaceDev s, const float *
 // element depends only on its own query row and code column, so both give
 // the same bits. QJL = false drops the s_q planes and their passes (queries
 // with s_q = 0: the symmetric MSE estimator, whose "queries" are decoded
-// codes); the QJL term is then exactly zero.
+// codes); the QJL term is then exactly zero. thr (coarse form, blocked
+// search): query q's running k-th best distance at thr[q * thr_stride], an
+// upper bound on its final one; near pairs that cannot reach it keep their
+// coarse value (refine_bound). thr = null: every near pair is corrected.
 template <bool NIB, int QT, bool QJL, bool CO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma(
     SpaceDev s, const _Float16 *__restrict__ planes, const float *__restrict__ prep, long m,
-    const uint8_t *__restrict__ codes, long n, float *__restrict__ out) {
+    const uint8_t *__restrict__ codes, long n, float *__restrict__ out,
+    const float *__restrict__ thr, long thr_stride) {
   using Sh = MfmaShape<NIB, QT, QJL, CO>;
   constexpr int RB = Sh::RB;
   static_assert(QT == 32 || QT == 64, "query tile");
@@ -120,8 +124,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   // entry k at 32 k + c), lane l reading copy l % 32 (decode8<true>).
   __shared__ uint32_t l_tab[NIB ? 1 : 128 * 32];
   __shared__ float l_qn[QT][2];  // q_norm_sq, q_norm of the tile's queries
-  // coarse form: the centroid remainders (8 or 128), for refine_flagged
+  // coarse form: the centroid remainders (8 or 128), for refine_flagged,
+  // and with thr the tile's thresholds and refine_limits
   __shared__ float l_clo[CO ? (NIB ? 8 : 128) : 1];
+  __shared__ float l_thr[CO ? QT : 1];
+  __shared__ float2 l_lim[CO ? QT : 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -151,8 +158,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     l_qn[tid][0] = pq[2 * s.D];
     l_qn[tid][1] = pq[2 * s.D + 1];
   }
-  if constexpr (CO)
+  const _Float16 *const corr = planes + split_planes_halves(s.D, m, Sh::NPL);
+  if constexpr (CO) {
     if (tid < (NIB ? 8 : 128)) l_clo[tid] = cent_lo(s, tid);
+    if (thr != nullptr && tid < QT) {
+      const long q = q0 + tid;
+      const float *nq = reinterpret_cast<const float *>(corr + corr_halves(s.D, m)) + 2 * q;
+      l_thr[tid] = q < m ? thr[q * thr_stride] : 0.0f;
+      l_lim[tid] = refine_limits(s, QJL, nq[0], nq[1]);
+    }
+  }
   if constexpr (!NIB) {
     for (int i = tid; i < 128 * 32; i += blockDim.x) {
       _Float16 ch, cl;
@@ -225,26 +240,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   else
     epilogue(std::false_type{});
   if constexpr (CO) {
+    // this lane's sums of element b (a select chain: no indexing by b)
+    auto sums = [&](int b, float &am, float &aq) {
+      am = 0.0f;
+      aq = 0.0f;
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int v = 0; v < 16; ++v)
+            if (b == 16 * (2 * r + c) + v) {
+              am = acc_m[r][c][v];
+              if constexpr (QJL) aq = acc_q[r][c][v];
+            }
+    };
+    if (thr != nullptr && __any(flag != 0u))
+      flag = prune_flagged(flag, [&](int b) {
+        float am, aq;
+        sums(b, am, aq);
+        const int v = b & 15, c = (b >> 4) & 1;
+        const int row = 32 * (b >> 5) + 8 * (v >> 2) + 4 * kg + (v & 3);
+        const float fmc = c ? fm[1] : fm[0], fqc = c ? fq[1] : fq[0];
+        const float xnc = c ? xnv[1] : xnv[0], xsc = c ? xns[1] : xns[0];
+        const float qns = l_qn[row][0], qn = l_qn[row][1];
+        const float d = mfma_distance<QJL>(am, aq, fmc, fqc, xnc, xsc, qns, qn);
+        const float e = refine_bound(am, aq, fmc, fqc, xnc, xsc, qns, qn, l_lim[row]);
+        return !(d - e > l_thr[row]);
+      });
     refine_flagged<NIB, QJL>(
-        s, [&](uint32_t k) { return l_clo[k]; }, planes + split_planes_halves(s.D, m, Sh::NPL), codes, flag, lane,
+        s, [&](uint32_t k) { return l_clo[k]; }, corr, codes, flag, lane,
         [&](int b, int src, long &q, long &j) {
           const int v = b & 15;
           q = q0 + 32 * (b >> 5) + 8 * (v >> 2) + 4 * (src >> 5) + (v & 3);
           j = j0 + wave * 64 + 32 * ((b >> 4) & 1) + (src & 31);
         },
         [&](int b, float dm, float dq) {
-          // this lane's sums of element b (a select chain: no indexing by b)
-          float am = 0.0f, aq = 0.0f;
-#pragma unroll
-          for (int r = 0; r < RB; ++r)
-#pragma unroll
-            for (int c = 0; c < 2; ++c)
-#pragma unroll
-              for (int v = 0; v < 16; ++v)
-                if (b == 16 * (2 * r + c) + v) {
-                  am = acc_m[r][c][v];
-                  if constexpr (QJL) aq = acc_q[r][c][v];
-                }
+          float am, aq;
+          sums(b, am, aq);
           const int v = b & 15, c = (b >> 4) & 1;
           const int row = 32 * (b >> 5) + 8 * (v >> 2) + 4 * kg + (v & 3);
           out[(q0 + row) * n + j0 + wave * 64 + 32 * c + lr] = mfma_distance<QJL>(
@@ -254,12 +287,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   }
 }
 
+// One wavefront per query row (m rounded up to 32; rows past m give 0):
+// sum |q_hi| and sum |s_lo| over the row's correction groups, each lane
+// summing its groups in order and the lanes folding in a fixed butterfly,
+// into norms[2 q], norms[2 q + 1].
+__global__ __launch_bounds__(256) void k_corr_norms(int D, long m,
+                                                    const _Float16 *__restrict__ corr,
+                                                    float *__restrict__ norms) {
+  const long q = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (q >= (m + 31) / 32 * 32) return;  // no barriers below
+  const int ng = D >> 3;
+  float a = 0.0f, b = 0.0f;
+  if (q < m)
+    for (int g = lane; g < ng; g += 64) {
+      const h8 qh = *reinterpret_cast<const h8 *>(corr + (q * ng + g) * kCorrHalves);
+      const h8 sl = *reinterpret_cast<const h8 *>(corr + (q * ng + g) * kCorrHalves + 8);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        a += __builtin_fabsf((float)qh[e]);
+        b += __builtin_fabsf((float)sl[e]);
+      }
+    }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o);
+    b += __shfl_xor(b, o);
+  }
+  if (lane == 0) {
+    norms[2 * q] = a;
+    norms[2 * q + 1] = b;
+  }
+}
+
 }  // namespace
 
 size_t split_planes_bytes(const SpaceDev &s, long m, bool qjl) {
   const bool co = mfma_coarse(s);
   long halves = split_planes_halves(s.D, m, split_planes(s.nib != 0, qjl, co));
-  if (co) halves += corr_halves(s.D, m);
+  if (co) halves += corr_halves(s.D, m) + 2 * corr_norm_floats(m);
   return (size_t)halves * sizeof(_Float16);
 }
 
@@ -271,33 +336,40 @@ hipError_t launch_split_queries(const SpaceDev &s, const float *prep, long m, _F
   if (blocks > 65535) blocks = 65535;
   const bool co = mfma_coarse(s);
   const int npl = split_planes(s.nib != 0, qjl, co);
+  _Float16 *corr = co ? planes + split_planes_halves(s.D, m, npl) : nullptr;
   hipLaunchKernelGGL(k_split_queries, dim3((unsigned)blocks), dim3(256), 0, st, s, prep, m,
-                     planes, npl, co ? planes + split_planes_halves(s.D, m, npl) : nullptr);
+                     planes, npl, corr);
+  if (co) {
+    const long rows = (m + 31) / 32 * 32;
+    hipLaunchKernelGGL(k_corr_norms, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st, s.D, m,
+                       corr, reinterpret_cast<float *>(corr + corr_halves(s.D, m)));
+  }
   return hipGetLastError();
 }
 
 namespace {
 template <int QT, bool QJL>
 void launch_tile(const SpaceDev &s, dim3 grid, const _Float16 *planes, const float *prep, long m,
-                 const uint8_t *codes, long n, float *out, hipStream_t st) {
+                 const uint8_t *codes, long n, float *out, const float *thr, long thr_stride,
+                 hipStream_t st) {
   if (mfma_coarse(s) && s.nib)
     hipLaunchKernelGGL((k_mfma<true, QT, QJL, true>), grid, dim3(256), 0, st, s, planes, prep, m,
-                       codes, n, out);
+                       codes, n, out, thr, thr_stride);
   else if (s.nib)
     hipLaunchKernelGGL((k_mfma<true, QT, QJL, false>), grid, dim3(256), 0, st, s, planes, prep, m,
-                       codes, n, out);
+                       codes, n, out, thr, thr_stride);
   else if (mfma_coarse(s))
     hipLaunchKernelGGL((k_mfma<false, QT, QJL, true>), grid, dim3(256), 0, st, s, planes, prep,
-                       m, codes, n, out);
+                       m, codes, n, out, thr, thr_stride);
   else
     hipLaunchKernelGGL((k_mfma<false, QT, QJL, false>), grid, dim3(256), 0, st, s, planes, prep,
-                       m, codes, n, out);
+                       m, codes, n, out, thr, thr_stride);
 }
 }  // namespace
 
 hipError_t launch_mfma_planes(const SpaceDev &s, const _Float16 *planes, const float *prep, long m,
                               const uint8_t *codes, long n, float *out, hipStream_t st, bool qjl,
-                              int qt) {
+                              int qt, const float *thr, long thr_stride) {
   if (s.D < kKC || m > 65535L * 64) return hipErrorNotSupported;
   if (m <= 0 || n <= 0) return hipSuccess;
   const long gx = (n + kTN - 1) / kTN;
@@ -307,18 +379,18 @@ hipError_t launch_mfma_planes(const SpaceDev &s, const _Float16 *planes, const f
     if (tiles > 0xFFFFFFFFL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)tiles);
     if (qjl)
-      launch_tile<32, true>(s, grid, planes, prep, m, codes, n, out, st);
+      launch_tile<32, true>(s, grid, planes, prep, m, codes, n, out, thr, thr_stride, st);
     else
-      launch_tile<32, false>(s, grid, planes, prep, m, codes, n, out, st);
+      launch_tile<32, false>(s, grid, planes, prep, m, codes, n, out, thr, thr_stride, st);
     return hipGetLastError();
   }
   const long tiles = gx * ((m + 63) / 64);
   if (tiles > 0xFFFFFFFFL) return hipErrorInvalidValue;
   const dim3 grid((unsigned)tiles);
   if (qjl)
-    launch_tile<64, true>(s, grid, planes, prep, m, codes, n, out, st);
+    launch_tile<64, true>(s, grid, planes, prep, m, codes, n, out, thr, thr_stride, st);
   else
-    launch_tile<64, false>(s, grid, planes, prep, m, codes, n, out, st);
+    launch_tile<64, false>(s, grid, planes, prep, m, codes, n, out, thr, thr_stride, st);
   return hipGetLastError();
 }
 

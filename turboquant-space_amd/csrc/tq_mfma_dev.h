@@ -46,6 +46,9 @@ constexpr int kCorrHalves = 16;  // halves per 8-coordinate group
 __host__ __device__ inline long corr_halves(int D, long m) {
   return (m + 31) / 32 * 32 * (long)(D / 8) * kCorrHalves;
 }
+// After the correction rows, two floats per padded row: sum |q_hi| and
+// sum |s_lo| of the row (k_corr_norms), for refine_limits.
+__host__ __device__ inline long corr_norm_floats(long m) { return (m + 31) / 32 * 32 * 2; }
 __host__ __device__ inline int split_perm(bool nib, int e) {
   return nib ? (e < 4 ? 2 * e : 2 * (e - 4) + 1) : e;
 }

@@ -261,6 +261,56 @@ __device__ __forceinline__ bool refine_flag(float d, float qns, float xns) {
   return d < kRefineFrac * (qns + xns);
 }
 
+// The search forms correct only the near pairs that can still reach their
+// query's top-k: a pair whose coarse distance d satisfies d - E > T, with T
+// an upper bound on the query's final k-th best distance (a running list's
+// k-th entry) and E >= |corrected d - d| (refine_bound), cannot enter the
+// top-k either way, so it keeps its coarse value and every form returns the
+// same top-k. E comes from bounds on the correction sums of the pair's query
+// (refine_limits): |dm| <= sum |q_hi| * max |c_lo| plus the fixed-point
+// rounding (2^-14 per 8-coordinate group), |dq| <= sum |s_lo| plus 2^-23 per
+// group (4-bit spaces with QJL only), each widened by 2^-10 for the fp32
+// sums behind them. Rows whose sums could leave the 32-bit fixed-point range
+// (never for prepared rows, |q| = 1) get an infinite bound: never pruned.
+__device__ __forceinline__ float2 refine_limits(const SpaceDev &s, bool sq, float l1_qhi,
+                                                float l1_slo) {
+  const float ng = (float)(s.D >> 3);
+  float dmb = (l1_qhi * s.clo_max + ng * 0x1p-14f) * (1.0f + 0x1p-10f);
+  const float dqb = sq ? (l1_slo + ng * 0x1p-23f) * (1.0f + 0x1p-10f) : 0.0f;
+  if (!(dmb < 0x1p17f && dqb < 0x1p8f)) dmb = __builtin_inff();
+  return make_float2(dmb, dqb);
+}
+
+// E for one pair: mfma_distance's change when am, aq move by at most lim.x,
+// lim.y (2 |dip|, dip = (dm fm + dq fq) xn qn), plus 2^-18 of the magnitudes
+// its fp32 steps round (the products, the sums and qns + xns: about 22 units
+// of 2^-24 in all, the final d - E comparison's included), the whole widened
+// by 2^-16 for this evaluation's own rounding. NaN or inf inputs give a NaN
+// or inf E, which never prunes.
+__device__ __forceinline__ float refine_bound(float am, float aq, float fm, float fq, float xn,
+                                              float xns, float qns, float qn, float2 lim) {
+  const float f = __builtin_fabsf(fm), g = __builtin_fabsf(fq);
+  const float t = (lim.x * f + lim.y * g) * xn * qn;
+  const float p = (__builtin_fabsf(am) * f + __builtin_fabsf(aq) * g) * xn * qn;
+  return (2.0f * t * (1.0f + 0x1p-18f) + 0x1p-18f * (p + qns + xns)) * (1.0f + 0x1p-16f);
+}
+
+// The bits of `flag` (one lane's flagged pairs) for which keep(b) holds;
+// keep runs in the bit's own lane, one bit per lane per step, until every
+// lane's bits are checked. Every lane of the wavefront must call it.
+template <typename Keep>
+__device__ __forceinline__ uint64_t prune_flagged(uint64_t flag, Keep &&keep) {
+  uint64_t pend = flag, kept = 0;
+  while (__any(pend != 0u)) {
+    if (pend != 0u) {
+      const int b = __builtin_ctzll(pend);
+      pend &= pend - 1u;
+      if (keep(b)) kept |= 1ull << b;
+    }
+  }
+  return kept;
+}
+
 // The terms the coarse passes drop for one (query, code) pair, in the MFMA
 // sums' units: dm = sum_i q_hi[i] * c_lo[idx_i], and for 4-bit codes with
 // QJL dq = sum_i s_lo[i] * (+-1)_i (byte codes keep no s_lo term in either

@@ -537,7 +537,9 @@ int enqueue(tq_search_plan_s *p, hipStream_t st) {
     }
     // Blocked batches: per chunk of codes, the distance block, its row top-k
     // as lists 1.. of each query, folded into list 0; the last fold writes
-    // the query's k best.
+    // the query's k best. After the first chunk, list 0's k-th entries bound
+    // each query's final k-th best: the coarse form's blocks correct only the
+    // near pairs that can still beat them.
     for (const PlanBatch &b : p->batches) {
       if (!b.blocked) continue;
       const long cs = (long)b.space->info.code_size;
@@ -545,8 +547,10 @@ int enqueue(tq_search_plan_s *p, hipStream_t st) {
         const long nc = std::min(b.nc, b.n - c0);
         const long l1 = 1 + tq::topk_rows_segments(b.m, nc);
         const bool last = c0 + nc >= b.n;
+        const float *thr = c0 > 0 ? p->seg_d + (p->k - 1) : nullptr;
         if ((e = tq::launch_mfma_planes(b.space->dev, b.planes, b.prep, b.m, b.codes + c0 * cs, nc,
-                                        p->dist, st, true, b.bqt)) != hipSuccess ||
+                                        p->dist, st, true, b.bqt, thr,
+                                        (long)b.lists * p->k)) != hipSuccess ||
             (e = tq::launch_topk_rows_chunk(p->dist, b.m, nc, b.lists, 1, (uint32_t)c0, p->k,
                                             c0 > 0, p->seg_d, p->seg_i, st)) != hipSuccess ||
             (e = tq::launch_topk_rows_fold(b.m, b.lists, c0 == 0 ? 1 : 0, l1, p->k, p->seg_d,

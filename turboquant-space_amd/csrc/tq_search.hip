@@ -128,8 +128,8 @@ template <bool NIB, bool CO, int QR, int K>
 __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, long j0, int kk,
                                             uint8_t *lds, const uint32_t *l_tab,
                                             const PermTab &ptab, const float (*l_qn)[2],
-                                            const float *l_clo, float (*l_ld)[K][64],
-                                            uint32_t (*l_li)[K][64]) {
+                                            const float *l_clo, const float2 *l_lim,
+                                            float (*l_ld)[K][64], uint32_t (*l_li)[K][64]) {
   constexpr int QT = 32 * QR;
   using Sh = MfmaShape<NIB, QT, true, CO>;
   const SpaceDev &s = job.s;
@@ -162,7 +162,9 @@ __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, lo
   // k_mfma's epilogue: the same distances, and the coarse form's flagged pairs
   // (bit 32 r + 16 c + v) corrected the same way, here by adding the dropped
   // terms to the sums before the distances are formed (the same fp32 sum
-  // k_mfma forms), all of the wavefront's blocks in one pass.
+  // k_mfma forms), all of the wavefront's blocks in one pass. Near pairs
+  // that cannot beat their list's kk-th entry (a bound on the query's final
+  // kk-th best) stay uncorrected, as in the blocked form (refine_bound).
   if constexpr (CO) {
     uint64_t flag = 0;
 #pragma unroll
@@ -179,8 +181,34 @@ __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, lo
             flag |= 1ull << (32 * r + 16 * c + v);
         }
       }
+    if (__any(flag != 0u))
+      flag = prune_flagged(flag, [&](int b) {
+        const int r = b >> 5, c = (b >> 4) & 1, v = b & 15;
+        float am = 0.0f, aq = 0.0f;  // a select chain: no indexing by b
+#pragma unroll
+        for (int rr = 0; rr < QR; ++rr)
+#pragma unroll
+          for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+            for (int vv = 0; vv < 16; ++vv)
+              if (b == 32 * rr + 16 * cc + vv) {
+                am = acc_m[rr][cc][vv];
+                aq = acc_q[rr][cc][vv];
+              }
+        const int qrow = 32 * r + 8 * (v >> 2) + 4 * kg + (v & 3);
+        // the list this pair would enter: lane qrow (QR = 2) or lane
+        // qrow + 32 c (QR = 1, lanes q and q + 32 split the codes)
+        const int owner = QR == 2 ? qrow : qrow + 32 * c;
+        const float fmc = c ? fm[1] : fm[0], fqc = c ? fq[1] : fq[0];
+        const float xnc = c ? xnv[1] : xnv[0], xsc = c ? xns[1] : xns[0];
+        const float qns = l_qn[qrow][0], qn = l_qn[qrow][1];
+        const float d = mfma_distance<true>(am, aq, fmc, fqc, xnc, xsc, qns, qn);
+        const float e = refine_bound(am, aq, fmc, fqc, xnc, xsc, qns, qn, l_lim[qrow]);
+        return !(d - e > l_ld[wave][kk - 1][owner]);
+      });
     refine_flagged<NIB, true>(
-        s, [&](uint32_t k) { return l_clo[k]; }, job.planes + split_planes_halves(s.D, m, Sh::NPL), job.codes, flag, lane,
+        s, [&](uint32_t k) { return l_clo[k]; }, job.planes + split_planes_halves(s.D, m, Sh::NPL),
+        job.codes, flag, lane,
         [&](int b, int src, long &q, long &j) {
           const int v = b & 15;
           q = q0 + 32 * (b >> 5) + 8 * (v >> 2) + 4 * (src >> 5) + (v & 3);
@@ -278,6 +306,7 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
   __shared__ uint32_t l_tab[128];
   __shared__ float l_qn[QT][2];
   __shared__ float l_clo[128];  // coarse jobs: the centroid remainders, for refine_flagged
+  __shared__ float2 l_lim[QT];  // coarse jobs: the tile's refine_limits
   __shared__ float l_ld[4][K][64];
   __shared__ uint32_t l_li[4][K][64];
   __shared__ int s_item;
@@ -310,6 +339,13 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
         l_tab[i] = h_bits(ch) | (h_bits(cl) << 16);
       }
     if (coarse && tid < (nib ? 8 : 128)) l_clo[tid] = cent_lo(s, tid);
+    if (coarse && tid < QT) {
+      const _Float16 *corr =
+          job.planes + split_planes_halves(s.D, m, split_planes(nib, true, true));
+      const float *nq = reinterpret_cast<const float *>(corr + corr_halves(s.D, m));
+      const long q = q0 + tid < m ? q0 + tid : 0L;
+      l_lim[tid] = refine_limits(s, nib, nq[2 * q], nq[2 * q + 1]);
+    }
     if (tid < QT) {
       const long q = q0 + tid;
       const float *pq = job.prep + (q < m ? q : 0L) * prep_stride(s.D);
@@ -328,17 +364,17 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
     for (int tile = item.t0; tile < item.t1; ++tile) {
       const long j0 = (long)tile * kSN;
       if (coarse && nib)
-        search_tile<true, true, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_ld,
-                                       l_li);
+        search_tile<true, true, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_lim,
+                                       l_ld, l_li);
       else if (nib)
-        search_tile<true, false, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_ld,
-                                        l_li);
+        search_tile<true, false, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_lim,
+                                        l_ld, l_li);
       else if (coarse)
-        search_tile<false, true, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_ld,
-                                        l_li);
+        search_tile<false, true, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_lim,
+                                        l_ld, l_li);
       else
-        search_tile<false, false, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_ld,
-                                         l_li);
+        search_tile<false, false, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_lim,
+                                         l_ld, l_li);
     }
 
     // This wavefront's lists for the item.
