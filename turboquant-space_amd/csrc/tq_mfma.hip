@@ -116,7 +116,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     SpaceDev s, const _Float16 *__restrict__ planes, const float *__restrict__ prep, long m,
     const uint8_t *__restrict__ codes, long n, float *__restrict__ out,
     const float *__restrict__ thr, long thr_stride) {
-  using Sh = MfmaShape<NIB, QT, QJL, CO>;
+  // 64-coordinate chunks (one barrier per 64) for the 64-query coarse tiles
+  // of 4-bit codes: their stage buffers still leave room for two
+  // workgroups per CU (2 x 64 KB of the 160 KB LDS).
+  constexpr bool WIDE = NIB && CO && QT == 64;
+  using Sh = MfmaShape<NIB, QT, QJL, CO, WIDE>;
   constexpr int RB = Sh::RB;
   static_assert(QT == 32 || QT == 64, "query tile");
   __shared__ __attribute__((aligned(16))) uint8_t stage[2][Sh::STAGE];
@@ -180,8 +184,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
   f16v acc_m[RB][2];
   f16v acc_q[QJL ? RB : 1][2];
-  mfma_tile<NIB, QT, QJL, true, CO>(s, planes, m, codes, n, q0, j0, stage, l_tab, ptab, acc_m,
-                                    acc_q);
+  mfma_tile<NIB, QT, QJL, true, CO, WIDE>(s, planes, m, codes, n, q0, j0, stage, l_tab, ptab,
+                                          acc_m, acc_q);
 
   // Epilogue: output element v of a 32x32 block sits at row
   // 8*(v/4) + 4*kg + (v%4), column lr. The per-code factors are formed once;

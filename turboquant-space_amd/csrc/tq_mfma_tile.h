@@ -18,20 +18,25 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
 
 // CO: the coarse pass set (mfma_coarse): no s_lo plane and no q_hi*c_lo
-// pass; the caller corrects the near pairs (refine_flagged).
-template <bool NIB, int QT, bool QJL, bool CO = false>
+// pass; the caller corrects the near pairs (refine_flagged). WIDE: chunks of
+// 64 coordinates (four k steps, one barrier per 64) instead of 32, for nibble
+// codes; a chunk is then two 32-coordinate halves laid out as before.
+template <bool NIB, int QT, bool QJL, bool CO = false, bool WIDE = false>
 struct MfmaShape {
+  static_assert(NIB || !WIDE, "wide chunks are for nibble codes");
+  static constexpr int KB = WIDE ? 4 : 2;                     // 16-coordinate k steps per chunk
   static constexpr int RB = QT / 32;                          // 32-row blocks per wavefront
   static constexpr int NPL = split_planes(NIB, QJL, CO);      // query planes read
-  static constexpr int NB = NPL * RB * 2;                     // 1 KB A blocks per chunk
+  static constexpr int NB = NPL * RB * KB;                    // 1 KB A blocks per chunk
   static constexpr int ALOADS = (NB * 64 + 255) / 256;        // 16-byte A loads per thread
-  static constexpr int NC = NIB ? 1 : 2;                      // 16-byte code loads per thread
+  static constexpr int NC = NIB ? KB / 2 : 2;                 // 16-byte code loads per thread
   static constexpr int A_BYTES = NB * 1024;
   static constexpr int STAGE = A_BYTES + 4 * NC * 1024;
   // Chunks whose loads are in the register ring: three where a chunk takes
-  // at most four 16-byte loads per thread, else two (the 64-query QJL tiles
-  // of 4-bit codes with every pass, and of byte codes).
-  static constexpr int RING = ALOADS + NC <= 4 ? 3 : 2;
+  // at most four 16-byte loads per thread, two up to six (the 64-query QJL
+  // tiles of 4-bit codes with every pass, and of byte codes), one for wide
+  // chunks (one chunk of lookahead = two 32-coordinate ones).
+  static constexpr int RING = ALOADS + NC <= 4 ? 3 : ALOADS + NC <= 6 ? 2 : 1;
 };
 
 // acc_m / acc_q of queries [q0, q0 + QT) x codes [j0, j0 + 256) of one
@@ -41,28 +46,26 @@ struct MfmaShape {
 // byte codes' centroid table (REP: replicated over 32 banks, decode8<true>)
 // and ptab the nibble codes' perm tables. Ends with a workgroup barrier after
 // the last chunk, so the caller may reuse the stage buffers.
-template <bool NIB, int QT, bool QJL, bool REP, bool CO>
-__device__ __forceinline__ void mfma_tile(const SpaceDev &s, const _Float16 *__restrict__ planes,
-                                          long m, const uint8_t *__restrict__ codes, long n,
-                                          long q0, long j0,
-                                          uint8_t (*stage)[MfmaShape<NIB, QT, QJL, CO>::STAGE],
-                                          const uint32_t *l_tab, const PermTab &ptab,
-                                          f16v (&acc_m)[QT / 32][2],
-                                          f16v (&acc_q)[QJL ? QT / 32 : 1][2]) {
-  using Sh = MfmaShape<NIB, QT, QJL, CO>;
-  constexpr int RB = Sh::RB, NPL = Sh::NPL, NB = Sh::NB, NC = Sh::NC;
+template <bool NIB, int QT, bool QJL, bool REP, bool CO, bool WIDE = false>
+__device__ __forceinline__ void mfma_tile(
+    const SpaceDev &s, const _Float16 *__restrict__ planes, long m,
+    const uint8_t *__restrict__ codes, long n, long q0, long j0,
+    uint8_t (*stage)[MfmaShape<NIB, QT, QJL, CO, WIDE>::STAGE], const uint32_t *l_tab,
+    const PermTab &ptab, f16v (&acc_m)[QT / 32][2], f16v (&acc_q)[QJL ? QT / 32 : 1][2]) {
+  using Sh = MfmaShape<NIB, QT, QJL, CO, WIDE>;
+  constexpr int RB = Sh::RB, NPL = Sh::NPL, NB = Sh::NB, NC = Sh::NC, KB = Sh::KB;
   constexpr int ALOADS = Sh::ALOADS, kRing = Sh::RING;
   static_assert(QT == 32 || QT == 64, "query tile");
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int lr = lane & 31, kg = lane >> 5;
-  const int nchunks = s.D / kKC;
+  const int nchunks = s.D / (16 * KB);
   const int nks = s.D / 16;
 
   // Per-thread load sources and LDS destinations, fixed for the whole walk
   // over the padded dim; a chunk only adds its offset. A chunk's A block
-  // i = (row block r, k step kb, plane p), i = (r * 2 + kb) * NPL + p, is
+  // i = (row block r, k step kb, plane p), i = (r * KB + kb) * NPL + p, is
   // 64 units of 16 bytes; thread t loads units t, t + 256, ... (a unit index
   // past the last loads the last unit again: same bytes, same place). Query
   // blocks past m read the last block, codes past n read code n - 1: every
@@ -74,7 +77,7 @@ __device__ __forceinline__ void mfma_tile(const SpaceDev &s, const _Float16 *__r
   for (int t = 0; t < ALOADS; ++t) {
     const int u = t * 256 + tid < NB * 64 ? t * 256 + tid : NB * 64 - 1;
     const int i = u >> 6, l = u & 63;
-    const int r = i / (2 * NPL), kb = (i / NPL) & 1, p = i % NPL;
+    const int r = i / (KB * NPL), kb = (i / NPL) % KB, p = i % NPL;
     long rb = q0 / 32 + r;
     if (rb >= nrb) rb = nrb - 1;
     asrc[t] = planes + ((rb * nks + kb) * NPL + p) * (long)kBlockHalves + 8 * l;
@@ -85,7 +88,7 @@ __device__ __forceinline__ void mfma_tile(const SpaceDev &s, const _Float16 *__r
   long jl = j0 + wave * 64 + lane;
   if (jl >= n) jl = n - 1;
   const uint8_t *csrc = codes + jl * (long)s.cs;
-  const long achunk = 2L * NPL * kBlockHalves;  // A halves per chunk
+  const long achunk = (long)KB * NPL * kBlockHalves;  // A halves per chunk
 
   u32x4 ring_a[kRing][ALOADS], ring_c[kRing][NC];
   auto load_group = [&](auto slot, int ch) __attribute__((always_inline)) {
@@ -97,7 +100,7 @@ __device__ __forceinline__ void mfma_tile(const SpaceDev &s, const _Float16 *__r
       ring_a[k][t] = *reinterpret_cast<const u32x4 *>(asrc[t] + cc * achunk);
 #pragma unroll
     for (int h = 0; h < NC; ++h)
-      ring_c[k][h] = *reinterpret_cast<const u32x4a4 *>(csrc + (NIB ? 16 : 32) * cc + 16 * h);
+      ring_c[k][h] = *reinterpret_cast<const u32x4a4 *>(csrc + (NIB ? 8 : 16) * KB * cc + 16 * h);
     __builtin_amdgcn_sched_barrier(0);
   };
   auto store_group = [&](auto slot, int b) __attribute__((always_inline)) {
@@ -128,13 +131,18 @@ __device__ __forceinline__ void mfma_tile(const SpaceDev &s, const _Float16 *__r
   auto compute = [&](int b) __attribute__((always_inline)) {
     const uint8_t *sa = &stage[b][0];
     const uint8_t *sc = &stage[b][Sh::A_BYTES + wave * NC * 1024];
-    uint32_t cw[2][4];
+    uint32_t cw[2][KB < 4 ? 4 : KB];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       if constexpr (NIB) {
-        const uint2 v = *reinterpret_cast<const uint2 *>(sc + (32 * c + lr) * 16 + 8 * kg);
-        cw[c][0] = v.x;
-        cw[c][1] = v.y;
+        // 32-coordinate half h of the chunk: its 16 code bytes in region h
+#pragma unroll
+        for (int h = 0; h < NC; ++h) {
+          const uint2 v =
+              *reinterpret_cast<const uint2 *>(sc + h * 1024 + (32 * c + lr) * 16 + 8 * kg);
+          cw[c][2 * h] = v.x;
+          cw[c][2 * h + 1] = v.y;
+        }
       } else {
         const uint4 v = *reinterpret_cast<const uint4 *>(sc + kg * 1024 + (32 * c + lr) * 16);
         cw[c][0] = v.x;
@@ -144,13 +152,14 @@ __device__ __forceinline__ void mfma_tile(const SpaceDev &s, const _Float16 *__r
       }
     }
 #pragma unroll
-    for (int kb = 0; kb < 2; ++kb) {
+    for (int kb = 0; kb < KB; ++kb) {
       h8 a[RB][NPL];
 #pragma unroll
       for (int r = 0; r < RB; ++r)
 #pragma unroll
         for (int p = 0; p < NPL; ++p)
-          a[r][p] = *reinterpret_cast<const h8 *>(sa + ((r * 2 + kb) * NPL + p) * 1024 + 16 * lane);
+          a[r][p] =
+              *reinterpret_cast<const h8 *>(sa + ((r * KB + kb) * NPL + p) * 1024 + 16 * lane);
       h8 bh[2], bl[2], bt[2];
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
@@ -196,7 +205,7 @@ __device__ __forceinline__ void mfma_tile(const SpaceDev &s, const _Float16 *__r
   using S1 = std::integral_constant<int, 1>;
   using S2 = std::integral_constant<int, kRing == 3 ? 2 : 0>;
   load_group(S0{}, 0);
-  load_group(S1{}, 1);
+  if constexpr (kRing >= 2) load_group(S1{}, 1);
   if constexpr (kRing == 3) load_group(S2{}, 2);
   store_group(S0{}, 0);
   load_group(S0{}, kRing);
@@ -212,12 +221,14 @@ __device__ __forceinline__ void mfma_tile(const SpaceDev &s, const _Float16 *__r
     }
     if (nchunks % 3 >= 1) step(S1{});
     if (nchunks % 3 >= 2) step(S2{});
-  } else {
+  } else if constexpr (kRing == 2) {
     for (int t = nchunks / 2; t > 0; --t) {
       step(S1{});
       step(S0{});
     }
     if (nchunks % 2) step(S1{});
+  } else {
+    for (int t = nchunks; t > 0; --t) step(S0{});
   }
 
 }
