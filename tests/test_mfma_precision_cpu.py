@@ -232,3 +232,140 @@ def test_every_kept_part_is_needed():
         assert worst[("byte", p)] > BAR, f"byte design without {p} still meets the bar"
     assert max(worst[("byte", "s_lo")], worst[("byte coarse", "s_lo")]) <= BAR / 2
     assert worst[("byte", "coarse")] > BAR / 2
+
+
+def _units(o, codes):
+    """Code units (index << 1 | QJL bit) of every coordinate."""
+    packed = codes[:, :o.packed]
+    if o.bits <= 4:
+        return np.stack([packed & 0xF, packed >> 4], axis=2).reshape(len(codes), -1)[:, :o.D]
+    return packed[:, :o.D].astype(np.int64)
+
+
+def correction_sums(qh, clo_u, sl, sgn, nib):
+    """refine_delta's totals for one query row against codes: per 8-coordinate
+    group (element order of the correction row) an fp32 fused multiply-add
+    chain of q_hi * c_lo and an fp32 sum of +-s_lo, each rounded to a 32-bit
+    fixed-point integer (2^13, 2^22), the integers summed, the totals scaled
+    back to fp32. clo_u, sgn: [codes, D] exact remainders and +-1 of the
+    codes' units."""
+    f32 = np.float32
+    D = qh.shape[-1]
+    perm = np.array([0, 2, 4, 6, 1, 3, 5, 7] if nib else list(range(8)))
+    order = (np.arange(D // 8)[:, None] * 8 + perm[None, :]).reshape(-1)
+    prod = (qh[None, order] * clo_u[:, order].astype(np.float64)).reshape(len(clo_u), -1, 8)
+    pm = np.zeros(prod.shape[:2], f32)
+    for e in range(8):
+        pm = (pm.astype(np.float64) + prod[:, :, e]).astype(f32)
+    dm = np.rint(np.ldexp(pm.astype(np.float64), 13)).sum(1).astype(f32) * f32(2.0 ** -13)
+    if not nib:
+        return dm, np.zeros(len(clo_u), f32)
+    t = (sl[None, order] * sgn[:, order]).reshape(len(clo_u), -1, 8)
+    pq = np.zeros(t.shape[:2], f32)
+    for e in range(8):
+        pq = (pq + t[:, :, e].astype(f32)).astype(f32)
+    dq = np.rint(np.ldexp(pq.astype(np.float64), 22)).sum(1).astype(f32) * f32(2.0 ** -22)
+    return dm, dq
+
+
+def refine_limits(D, l1_qhi, l1_slo, clo_max, nib):
+    """tq_mfma_tile.h refine_limits: bounds on |dm| and |dq| of a query row."""
+    f32 = np.float32
+    ng = f32(D // 8)
+    dmb = (f32(l1_qhi) * f32(clo_max) + ng * f32(2.0 ** -14)) * f32(1 + 2.0 ** -10)
+    dqb = (f32(l1_slo) + ng * f32(2.0 ** -23)) * f32(1 + 2.0 ** -10) if nib else f32(0)
+    return dmb, dqb
+
+
+def coarse_tables(o):
+    """Scaled centroids' fp16 parts and exact fp32 remainders (cent_lo), and
+    the largest remainder the code units can index (SpaceDev::clo_max)."""
+    f32 = np.float32
+    alpha = coarse_alpha(o.centroids)[0] if o.bits <= 4 else 256.0
+    cs = (o.centroids * f32(alpha)).astype(f32)
+    chi = cs.astype(np.float16).astype(f32)
+    clo = (cs - chi).astype(f32)
+    return alpha, chi.astype(np.float64), clo, float(np.abs(clo[:min(len(clo), 128)]).max())
+
+
+@pytest.mark.parametrize("bits", [4, 8])
+def test_correction_limits_hold_on_a_worst_case_row(bits):
+    """refine_limits bounds |dm| by sum |q_hi| * max |c_lo|: a query row whose
+    signs all match the remainder of the one centroid every coordinate of the
+    code uses reaches that bound to within 1%, so a bound from the Euclidean
+    norm of q_hi (32x smaller at D = 1024) would fail here; likewise |dq| <=
+    sum |s_lo| when every QJL sign matches s_lo's."""
+    f32 = np.float32
+    o = Oracle(768, bits)
+    D, nib = o.D, bits <= 4
+    _, _, clo, clo_max = coarse_tables(o)
+    kmax = int(np.argmax(np.abs(clo[:min(len(clo), 128)])))
+    rng = np.random.default_rng(5)
+    q = (np.sign(clo[kmax]) * rng.uniform(0.5, 1.5, D)).astype(f32)
+    q /= np.linalg.norm(q)
+    qh = split16(q * f32(1024))[0]
+    s = rng.standard_normal(D).astype(f32)
+    s /= np.linalg.norm(s)
+    sl = split16(s * f32(1024))[1]
+    clo_u = np.full((1, D), clo[kmax], f32)
+    sgn = np.sign(sl)[None, :]
+    sgn[sgn == 0] = 1.0
+    dm, dq = correction_sums(qh, clo_u, sl, sgn, nib)
+    dmb, dqb = refine_limits(D, np.abs(qh).sum(), np.abs(sl).sum(), clo_max, nib)
+    assert abs(dm[0]) <= dmb and abs(dm[0]) >= 0.99 * dmb
+    assert np.sqrt((qh ** 2).sum()) * clo_max < 0.05 * abs(dm[0])
+    if nib:
+        assert abs(dq[0]) <= dqb and abs(dq[0]) >= 0.99 * dqb
+
+
+@pytest.mark.parametrize("label,dim,bits,k,seed", [s for s in SAMPLES if s[1] >= 768])
+def test_pruning_bound_covers_the_correction(label, dim, bits, k, seed):
+    """The search forms leave a near pair uncorrected when its coarse distance
+    minus E (refine_bound, tq_mfma_tile.h) exceeds a bound on the query's
+    k-th best; that is exact only if E >= |corrected d - coarse d| for every
+    pair. This rebuilds the device arithmetic in fp32 (the coarse sums, the
+    correction totals, the epilogue's distance) and E from the query's L1
+    norms and the space's largest centroid remainder, and checks the
+    inequality on every pair of the coarse-form samples (and |dm|, |dq|
+    against their limits)."""
+    o, Q, codes, d64, idx = sample(dim, bits, k, seed)
+    assert design(o) == {"q_lo", "refine"}
+    f32 = np.float32
+    D, nib = o.D, o.bits <= 4
+    prep = o.prepare(Q)
+    q, s = prep[:, :D], prep[:, D:2 * D]
+    qns, qn = prep[:, 2 * D], prep[:, 2 * D + 1]
+    tr = o.trailer(codes)
+    xn, gamma, sigma = tr[:, 0], tr[:, 1], tr[:, 2]
+    alpha, chi, clo, clo_max = coarse_tables(o)
+    qh, ql = split16(q * f32(1024))
+    sh, sl = split16(s * f32(1024))
+    units = _units(o, codes)
+    idx_u, sgn = units >> 1, np.where(units & 1, 1.0, -1.0)
+    am_c = ((qh + ql) @ chi[idx_u].T).astype(f32)
+    aq_c = (sh @ sgn.T).astype(f32)
+    fm = (sigma * f32(1.0 / (1024.0 * alpha))).astype(f32)
+    fq = (f32(o.scale) * gamma * f32(1.0 / 1024.0)).astype(f32)
+    xns = (xn * xn).astype(f32)
+
+    def dist(am, aq, i):
+        ip = (am * fm + aq * fq) * xn * qn[i]
+        return np.maximum((qns[i] + xns) - f32(2) * ip, f32(0))
+
+    worst = 0.0
+    for i in range(len(Q)):
+        dm, dq = correction_sums(qh[i], clo[idx_u], sl[i], sgn, nib)
+        dmb, dqb = refine_limits(D, np.abs(qh[i]).sum(), np.abs(sl[i]).sum(), clo_max, nib)
+        assert (np.abs(dm) <= dmb).all() and (np.abs(dq) <= dqb).all()
+        d_c = dist(am_c[i], aq_c[i], i)
+        d_r = dist((am_c[i] + dm).astype(f32), (aq_c[i] + dq).astype(f32), i)
+        fa, ga = np.abs(fm), np.abs(fq)
+        tt = (dmb * fa + dqb * ga) * xn * qn[i]
+        pp = (np.abs(am_c[i]) * fa + np.abs(aq_c[i]) * ga) * xn * qn[i]
+        E = (f32(2) * tt * f32(1 + 2.0 ** -18)
+             + f32(2.0 ** -18) * (pp + qns[i] + xns)) * f32(1 + 2.0 ** -16)
+        diff = np.abs(d_r.astype(np.float64) - d_c.astype(np.float64))
+        assert (diff <= E).all(), f"{label}: query {i}: bound exceeded by {(diff - E).max():.3e}"
+        worst = max(worst, float((diff / E).max()))
+    print(f"{label}: max |corrected - coarse| / E = {worst:.3f}")
+    assert worst > 1e-3  # the check compares against a correction that is really there
