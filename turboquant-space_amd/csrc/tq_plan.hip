@@ -153,7 +153,7 @@ struct tq_search_plan_s {
   // workspace layout (byte offsets) of the current build
   size_t o_jobs = 0, o_items = 0, o_order = 0, o_groups = 0, o_gather = 0, o_scatter = 0,
          o_counter = 0, o_cand_d = 0, o_cand_i = 0, o_all_d = 0, o_all_i = 0, o_dist = 0,
-         o_seg_d = 0, o_seg_i = 0, o_scan_ws = 0, scan_ws_bytes = 0, need = 0;
+         o_seg_d = 0, o_seg_i = 0, o_scan_ws = 0, o_thr = 0, scan_ws_bytes = 0, need = 0;
   std::vector<size_t> o_q, o_prep, o_planes;
 
   // device workspace: owned (tq_search_plan_create) or lent by the caller
@@ -169,6 +169,7 @@ struct tq_search_plan_s {
   int *d_counter = nullptr;
   float *cand_d = nullptr;
   uint32_t *cand_i = nullptr;
+  float *thr = nullptr;  // fused form: running k-th best bound per query row
   float *all_d = nullptr;
   int64_t *all_i = nullptr;
   // FP32 plans
@@ -369,6 +370,7 @@ int layout(tq_search_plan_s *p) {
   const size_t cand = p->h_items.size() * 4 * 64 * (size_t)tq::kSearchMaxK;
   p->o_cand_d = take(cand * sizeof(float));
   p->o_cand_i = take(cand * sizeof(uint32_t));
+  p->o_thr = take((size_t)p->rows * sizeof(float));
   p->o_all_d = take((size_t)p->rows * p->k * sizeof(float));
   p->o_all_i = take((size_t)p->rows * p->k * sizeof(int64_t));
   p->o_q.assign(nb, 0);
@@ -424,6 +426,7 @@ int bind(tq_search_plan_s *p, void *ws, hipStream_t st, bool async) {
   p->d_counter = reinterpret_cast<int *>(w + p->o_counter);
   p->cand_d = reinterpret_cast<float *>(w + p->o_cand_d);
   p->cand_i = reinterpret_cast<uint32_t *>(w + p->o_cand_i);
+  p->thr = reinterpret_cast<float *>(w + p->o_thr);
   p->all_d = reinterpret_cast<float *>(w + p->o_all_d);
   p->all_i = reinterpret_cast<int64_t *>(w + p->o_all_i);
   p->dist = reinterpret_cast<float *>(w + p->o_dist);
@@ -451,6 +454,7 @@ int bind(tq_search_plan_s *p, void *ws, hipStream_t st, bool async) {
     jd.n = b.n;
     jd.out_d = p->all_d + b.row0 * p->k;
     jd.out_i = p->all_i + b.row0 * p->k;
+    jd.thr = p->thr + b.row0;
     p->h_jobs.push_back(jd);
     const long dim = (long)b.space->info.dim;
     for (size_t t = 0; t < b.jobs.size(); ++t) {
@@ -530,6 +534,8 @@ int enqueue(tq_search_plan_s *p, hipStream_t st) {
   if (p->mode == TQ_MODE_MFMA) {
     // Fused batches (none when every batch runs blocked, as for k > 16).
     if (!p->h_items.empty() || !p->h_groups.empty()) {
+      if ((e = tq::launch_fill_f32(p->thr, (long)p->rows, __builtin_inff(), st)) != hipSuccess)
+        return hip_fail(e, what);
       e = tq::launch_search(p->d_jobs, p->d_items, p->d_order, p->nitems_qr, p->d_groups,
                             (int)p->h_groups.size(), p->k, p->d_counter, p->cand_d, p->cand_i,
                             p->grid_qr, st);

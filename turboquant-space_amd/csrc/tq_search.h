@@ -19,6 +19,10 @@ struct SearchJobDev {
   long m, n;
   float *out_d;            // [m][k]
   int64_t *out_i;          // [m][k]; -1 where fewer than k codes exist
+  // [m] the fused form's running bound on each query's k-th best distance
+  // (+inf before the launch, lowered by atomicMin on the bits of
+  // non-negative floats), for the coarse form's pruning
+  float *thr;
 };
 
 // Code tiles [t0, t1) of query tile `qtile` (of 32 * qr queries) of job `job`.
@@ -33,6 +37,8 @@ struct SearchGroup {
 };
 
 int search_max_k();
+// p[0 .. n) = v (a kernel, so graph captures hold no memset node).
+hipError_t launch_fill_f32(float *p, long n, float v, hipStream_t st);
 // Per query-tile kind (qr = 1: 32 queries, qr = 2: 64 queries): resets that
 // kind's queue counter (d_counter[qr - 1]) and runs a persistent search over
 // grid_qr[qr - 1] workgroups on the nitems_qr[qr - 1] items listed next in

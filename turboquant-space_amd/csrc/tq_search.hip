@@ -105,6 +105,12 @@ __device__ __forceinline__ void topk_clear(float (&td)[K], uint32_t (&ti)[K]) {
 
 __global__ void k_reset_counter(int *counter) { *counter = 0; }
 
+__global__ __launch_bounds__(256) void k_fill_f32(float *__restrict__ p, long n, float v) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 template <int QR>
 struct SearchShape {
   static constexpr int QT = 32 * QR;  // queries per item
@@ -129,7 +135,8 @@ __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, lo
                                             uint8_t *lds, const uint32_t *l_tab,
                                             const PermTab &ptab, const float (*l_qn)[2],
                                             const float *l_clo, const float2 *l_lim,
-                                            float (*l_ld)[K][64], uint32_t (*l_li)[K][64]) {
+                                            float (*l_gthr)[64], float (*l_ld)[K][64],
+                                            uint32_t (*l_li)[K][64]) {
   constexpr int QT = 32 * QR;
   using Sh = MfmaShape<NIB, QT, true, CO>;
   const SpaceDev &s = job.s;
@@ -142,6 +149,13 @@ __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, lo
   auto t_lds = reinterpret_cast<float(*)[32][kTRow]>(lds);
 
   f16v acc_m[QR][2], acc_q[QR][2];
+  // Coarse form: the job's running bound on the k-th best of query row `grow`
+  // (lane l holds row l, or l % 32 for 32-query tiles), loaded now and
+  // consumed after the contraction.
+  const int grow = QR == 2 ? lane : (lane & 31);
+  float gth = __builtin_inff();
+  if constexpr (CO)
+    if (q0 + grow < m) gth = job.thr[q0 + grow];
   __syncthreads();  // the previous tile's transpose reads are done with the stage area
   mfma_tile<NIB, QT, true, false, CO>(s, job.planes, m, job.codes, n, q0, j0, stage, l_tab, ptab,
                                       acc_m, acc_q);
@@ -166,6 +180,7 @@ __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, lo
   // that cannot beat their list's kk-th entry (a bound on the query's final
   // kk-th best) stay uncorrected, as in the blocked form (refine_bound).
   if constexpr (CO) {
+    l_gthr[wave][lane] = gth;  // the wavefront's own copy: no barrier needed
     uint64_t flag = 0;
 #pragma unroll
     for (int r = 0; r < QR; ++r)
@@ -204,7 +219,7 @@ __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, lo
         const float qns = l_qn[qrow][0], qn = l_qn[qrow][1];
         const float d = mfma_distance<true>(am, aq, fmc, fqc, xnc, xsc, qns, qn);
         const float e = refine_bound(am, aq, fmc, fqc, xnc, xsc, qns, qn, l_lim[qrow]);
-        return !(d - e > l_ld[wave][kk - 1][owner]);
+        return !(d - e > __builtin_fminf(l_ld[wave][kk - 1][owner], l_gthr[wave][qrow]));
       });
     refine_flagged<NIB, true>(
         s, [&](uint32_t k) { return l_clo[k]; }, job.planes + split_planes_halves(s.D, m, Sh::NPL),
@@ -284,6 +299,26 @@ __device__ __forceinline__ void search_tile(const SearchJobDev &job, long q0, lo
     }
     __syncthreads();  // block r's transpose reads are done before block r + 1 is written
   }
+  // Publish a bound on each query's k-th best from the workgroup's lists of
+  // it (4 lists, one per wavefront, or 8 with 32-query tiles, over disjoint
+  // codes): with r = ceil(kk / lists), the largest of the lists' r-th
+  // entries has at least kk distinct codes at or below it. One thread per
+  // query row; distances are never negative or NaN, so their bits order as
+  // unsigned integers.
+  if constexpr (CO) {
+    if (tid < QT) {
+      constexpr int NL = QR == 2 ? 4 : 8;
+      const int r = (kk + NL - 1) / NL;
+      float bnd = 0.0f;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        bnd = __builtin_fmaxf(bnd, l_ld[w][r - 1][tid]);
+        if constexpr (QR == 1) bnd = __builtin_fmaxf(bnd, l_ld[w][r - 1][tid + 32]);
+      }
+      if (q0 + tid < m && bnd < gth)
+        atomicMin(reinterpret_cast<unsigned int *>(job.thr) + q0 + tid, __float_as_uint(bnd));
+    }
+  }
 }
 
 // Persistent fused search: workgroups pull (job, query tile, code-tile
@@ -307,6 +342,7 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
   __shared__ float l_qn[QT][2];
   __shared__ float l_clo[128];  // coarse jobs: the centroid remainders, for refine_flagged
   __shared__ float2 l_lim[QT];  // coarse jobs: the tile's refine_limits
+  __shared__ float l_gthr[4][64];  // coarse jobs: each wavefront's copy of the job-wide bounds
   __shared__ float l_ld[4][K][64];
   __shared__ uint32_t l_li[4][K][64];
   __shared__ int s_item;
@@ -365,16 +401,16 @@ k_search(const SearchJobDev *__restrict__ jobs, const SearchItem *__restrict__ i
       const long j0 = (long)tile * kSN;
       if (coarse && nib)
         search_tile<true, true, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_lim,
-                                       l_ld, l_li);
+                                       l_gthr, l_ld, l_li);
       else if (nib)
         search_tile<true, false, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_lim,
-                                        l_ld, l_li);
+                                        l_gthr, l_ld, l_li);
       else if (coarse)
         search_tile<false, true, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_lim,
-                                        l_ld, l_li);
+                                        l_gthr, l_ld, l_li);
       else
         search_tile<false, false, QR, K>(job, q0, j0, kk, lds, l_tab, ptab, l_qn, l_clo, l_lim,
-                                         l_ld, l_li);
+                                         l_gthr, l_ld, l_li);
     }
 
     // This wavefront's lists for the item.
@@ -808,6 +844,13 @@ __global__ __launch_bounds__(256) void k_topk_rows_fold(long m, long stride, lon
 }  // namespace
 
 int search_max_k() { return kSearchMaxK; }
+
+hipError_t launch_fill_f32(float *p, long n, float v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const long blocks = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
+  hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
 
 hipError_t launch_search(const SearchJobDev *d_jobs, const SearchItem *d_items,
                          const int *d_order, const int *nitems_qr, const SearchGroup *d_groups,

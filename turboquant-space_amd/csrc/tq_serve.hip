@@ -228,9 +228,13 @@ size_t batch_bound(tq_server_s *s, const std::vector<Request *> &b) {
     bytes += align_up(r->rows * in.dim * sizeof(float)) + 2048;
     if (r->kind != kSearch) continue;
     any = true;
-    // split query planes: at most 4 fp16 planes over rows rounded up to 32
-    bytes += r->rows * (in.dim * 4 + in.prepared_stride * 4 + in.padded_dim * 8 +
-                        (size_t)s->cfg.k * 24) + 31 * in.padded_dim * 8 + 8 * 256;
+    // split query planes over rows rounded up to 32: at most 5 D halves per
+    // row (4 planes, or the coarse form's 3 planes and its 2 D-halves
+    // correction row) plus the correction row's two norms; the fused form's
+    // running threshold per row
+    const size_t planes_row = in.padded_dim * 10 + 8;
+    bytes += r->rows * (in.dim * 4 + in.prepared_stride * 4 + planes_row + 4 +
+                        (size_t)s->cfg.k * 24) + 31 * planes_row + 8 * 256;
     qtiles += (r->rows + 31) / 32;
   }
   // Searches the plan runs plain (fp32 scan in row blocks + row top-k): all of
