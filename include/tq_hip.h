@@ -244,8 +244,9 @@ typedef struct tq_search_plan_s *tq_search_plan;
  * the batch against a chunk of codes into a scratch block of at most 256 MB
  * or 4,096 columns, a row top-k per chunk, candidates folded per query), or
  * fused (every fused batch of the plan in one persistent launch over a
- * (query tile, code range) queue, the top-k lists kept in registers); both
- * give the same results. k > TQ_FUSED_MAX_K always runs blocked (a fused
+ * (query tile, code range) queue, the top-k lists kept in LDS); both give
+ * the top-k of the batch's full MFMA distance matrix (tq_distance_m_to_n),
+ * bit for bit. k > TQ_FUSED_MAX_K always runs blocked (a fused
  * `tuning` is then refused, as are MFMA plans of more than 65535 queries).
  * TQ_MODE_FP32: per job the plain fp32 distance matrix in row blocks, then a
  * row top-k. `tuning` may be NULL. */
