@@ -184,8 +184,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
   f16v acc_m[RB][2];
   f16v acc_q[QJL ? RB : 1][2];
-  mfma_tile<NIB, QT, QJL, true, CO, WIDE>(s, planes, m, codes, n, q0, j0, stage, l_tab, ptab,
-                                          acc_m, acc_q);
+  mfma_tile<NIB, QT, QJL, true, CO, WIDE, true>(s, planes, m, codes, n, q0, j0, stage, l_tab,
+                                                ptab, acc_m, acc_q);
 
   // Epilogue: output element v of a 32x32 block sits at row
   // 8*(v/4) + 4*kg + (v%4), column lr. The per-code factors are formed once;

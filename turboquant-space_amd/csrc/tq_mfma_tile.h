@@ -46,7 +46,7 @@ struct MfmaShape {
 // byte codes' centroid table (REP: replicated over 32 banks, decode8<true>)
 // and ptab the nibble codes' perm tables. Ends with a workgroup barrier after
 // the last chunk, so the caller may reuse the stage buffers.
-template <bool NIB, int QT, bool QJL, bool REP, bool CO, bool WIDE = false>
+template <bool NIB, int QT, bool QJL, bool REP, bool CO, bool WIDE = false, bool RUNS = false>
 __device__ __forceinline__ void mfma_tile(
     const SpaceDev &s, const _Float16 *__restrict__ planes, long m,
     const uint8_t *__restrict__ codes, long n, long q0, long j0,
@@ -83,11 +83,22 @@ __device__ __forceinline__ void mfma_tile(
     asrc[t] = planes + ((rb * nks + kb) * NPL + p) * (long)kBlockHalves + 8 * l;
     adst[t] = u * 16;
   }
-  // Code bytes: thread (w, l) loads 16-byte part h of the chunk's bytes of
-  // code j0 + 64 w + l into LDS region (w * NC + h) at 16 l.
-  long jl = j0 + wave * 64 + lane;
-  if (jl >= n) jl = n - 1;
-  const uint8_t *csrc = codes + jl * (long)s.cs;
+  // Code bytes: 16-byte part h of the chunk's bytes of code j0 + 64 w + c
+  // goes to LDS region (w * NC + h) at 16 c. Load u of thread (w, l) takes
+  // part u of code l, or with RUNS (and NC parts) part l % NC of code
+  // (64 / NC) u + l / NC, so that one load instruction reads whole 16 NC-byte
+  // runs of 64 / NC codes (fewer cache lines per instruction than 16 bytes of
+  // each of 64 codes; 1.5% on the 64 x 10M dim-768 scan, but two more
+  // pointers than the fused search's registers can spare).
+  constexpr bool R = RUNS && NC > 1;
+  const int cpart = R ? lane % NC : 0;
+  const uint8_t *csrc[NC];
+#pragma unroll
+  for (int u = 0; u < NC; ++u) {
+    long jl = j0 + wave * 64 + (R ? (64 / NC) * u + lane / NC : lane);
+    if (jl >= n) jl = n - 1;
+    csrc[u] = codes + jl * (long)s.cs + 16 * (R ? cpart : u);
+  }
   const long achunk = (long)KB * NPL * kBlockHalves;  // A halves per chunk
 
   u32x4 ring_a[kRing][ALOADS], ring_c[kRing][NC];
@@ -100,7 +111,7 @@ __device__ __forceinline__ void mfma_tile(
       ring_a[k][t] = *reinterpret_cast<const u32x4 *>(asrc[t] + cc * achunk);
 #pragma unroll
     for (int h = 0; h < NC; ++h)
-      ring_c[k][h] = *reinterpret_cast<const u32x4a4 *>(csrc + (NIB ? 8 : 16) * KB * cc + 16 * h);
+      ring_c[k][h] = *reinterpret_cast<const u32x4a4 *>(csrc[h] + (NIB ? 8 : 16) * KB * cc);
     __builtin_amdgcn_sched_barrier(0);
   };
   auto store_group = [&](auto slot, int b) __attribute__((always_inline)) {
@@ -109,9 +120,12 @@ __device__ __forceinline__ void mfma_tile(
     for (int t = 0; t < ALOADS; ++t)
       *reinterpret_cast<u32x4 *>(&stage[b][adst[t]]) = ring_a[k][t];
 #pragma unroll
-    for (int h = 0; h < NC; ++h)
-      *reinterpret_cast<u32x4 *>(&stage[b][Sh::A_BYTES + (wave * NC + h) * 1024 + 16 * lane]) =
-          ring_c[k][h];
+    for (int u = 0; u < NC; ++u) {
+      const int c = R ? (64 / NC) * u + lane / NC : lane;
+      const int part = R ? cpart : u;
+      *reinterpret_cast<u32x4 *>(&stage[b][Sh::A_BYTES + (wave * NC + part) * 1024 + 16 * c]) =
+          ring_c[k][u];
+    }
   };
 
 #pragma unroll
