@@ -116,10 +116,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     SpaceDev s, const _Float16 *__restrict__ planes, const float *__restrict__ prep, long m,
     const uint8_t *__restrict__ codes, long n, float *__restrict__ out,
     const float *__restrict__ thr, long thr_stride) {
-  // 64-coordinate chunks (one barrier per 64) for the 64-query coarse tiles
-  // of 4-bit codes: their stage buffers still leave room for two
-  // workgroups per CU (2 x 64 KB of the 160 KB LDS).
-  constexpr bool WIDE = NIB && CO && QT == 64;
+  // 64-coordinate chunks (one barrier per 64) for nibble codes where the
+  // stage buffers still leave room for two workgroups per CU: the 64-query
+  // coarse tiles (2 x 64 KB of the 160 KB LDS) and every 32-query tile.
+  constexpr bool WIDE = NIB && (CO || QT == 32);
   using Sh = MfmaShape<NIB, QT, QJL, CO, WIDE>;
   constexpr int RB = Sh::RB;
   static_assert(QT == 32 || QT == 64, "query tile");
