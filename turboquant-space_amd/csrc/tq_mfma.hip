@@ -125,8 +125,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   static_assert(QT == 32 || QT == 64, "query tile");
   __shared__ __attribute__((aligned(16))) uint8_t stage[2][Sh::STAGE];
   // Byte codes: 32 copies of each of the 128 centroid entries (copy c of
-  // entry k at 32 k + c), lane l reading copy l % 32 (decode8<true>).
-  __shared__ uint32_t l_tab[NIB ? 1 : 128 * 32];
+  // entry k at 32 k + c), lane l reading copy l % 32 (decode8<true>). Its
+  // alignment to its own 16 KB size puts it at LDS offset 0, so an entry's
+  // address is the or of the entry and copy offsets (one and-or per byte).
+  __shared__ __attribute__((aligned(NIB ? 4 : 16384))) uint32_t l_tab[NIB ? 1 : 128 * 32];
   __shared__ float l_qn[QT][2];  // q_norm_sq, q_norm of the tile's queries
   // coarse form: the centroid remainders (8 or 128), for refine_flagged,
   // and with thr the tile's thresholds and refine_limits

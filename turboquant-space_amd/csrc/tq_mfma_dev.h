@@ -138,27 +138,44 @@ __device__ __forceinline__ void decode_nib_word(uint32_t w, const PermTab &t, h8
   }
 }
 
-// Units of 8 consecutive coordinates -> B-operand fragments. l_tab holds
-// (c_hi | c_lo << 16) of centroid k at entry k (REP = false), or 32 copies of
-// it at entries 32 k + c (REP = true) of which lane l reads copy c = l % 32,
-// so that a half-wavefront's lookups fall in 32 distinct banks.
+// Units of 8 consecutive coordinates -> B-operand fragments, from the two
+// code words holding their bytes (w0: coordinates 0-3, w1: 4-7, low byte
+// first). l_tab holds (c_hi | c_lo << 16) of centroid k at entry k
+// (REP = false), or 32 copies of it at entries 32 k + c (REP = true) of
+// which lane l reads copy c = l % 32, so that a half-wavefront's lookups fall
+// in 32 distinct banks. Each byte's entry offset is one shift and one
+// and-or of its word; the lookups are packed into fp16 pairs by byte
+// permutes, and the +-1 plane is built as in decode_nib_word.
 template <bool REP = false>
-__device__ __forceinline__ void decode8(const uint32_t (&u)[8], const uint32_t *l_tab, h8 &bh,
+__device__ __forceinline__ void decode8(uint32_t w0, uint32_t w1, const uint32_t *l_tab, h8 &bh,
                                         h8 &bl, h8 &bt, uint32_t copy = 0) {
+  constexpr int S = REP ? 7 : 2;  // log2 of the entry stride in bytes
+  const uint8_t *tab = reinterpret_cast<const uint8_t *>(l_tab);
+  const uint32_t w[2] = {w0, w1};
   uint32_t ph[4], pl[4], pt[4];
 #pragma unroll
-  for (int p = 0; p < 4; ++p) {
-    const uint32_t t0 = l_tab[REP ? ((u[2 * p] >> 1) << 5) | copy : u[2 * p] >> 1];
-    const uint32_t t1 = l_tab[REP ? ((u[2 * p + 1] >> 1) << 5) | copy : u[2 * p + 1] >> 1];
-    ph[p] = (t0 & 0xFFFFu) | (t1 << 16);
-    pl[p] = (t0 >> 16) | (t1 & 0xFFFF0000u);
-    pt[p] = 0xBC00BC00u ^ ((u[2 * p] & 1u) << 15) ^ ((u[2 * p + 1] & 1u) << 31);
+  for (int h = 0; h < 2; ++h) {
+    uint32_t t[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      // centroid index = bits 8 b + 1 .. 8 b + 7 of the word, moved to bit S
+      const int sh = 8 * b + 1 - S;
+      const uint32_t off = (sh >= 0 ? w[h] >> sh : w[h] << -sh) & (127u << S);
+      t[b] = *reinterpret_cast<const uint32_t *>(tab + (REP ? (off | (copy << 2)) : off));
+    }
+    ph[2 * h] = __builtin_amdgcn_perm(t[1], t[0], 0x05040100u);
+    pl[2 * h] = __builtin_amdgcn_perm(t[1], t[0], 0x07060302u);
+    ph[2 * h + 1] = __builtin_amdgcn_perm(t[3], t[2], 0x05040100u);
+    pl[2 * h + 1] = __builtin_amdgcn_perm(t[3], t[2], 0x07060302u);
+    // bit 0 of each byte: 1 -> +1 (0x3C00), 0 -> -1 (0xBC00)
+    const uint32_t T = __builtin_amdgcn_perm(0u, 0x3CBCu, w[h] & 0x01010101u);
+    pt[2 * h] = __builtin_amdgcn_perm(T, 0u, 0x050C040Cu);
+    pt[2 * h + 1] = __builtin_amdgcn_perm(T, 0u, 0x070C060Cu);
   }
   bh = __builtin_bit_cast(h8, ph);
   bl = __builtin_bit_cast(h8, pl);
   bt = __builtin_bit_cast(h8, pt);
 }
-
 
 }  // namespace mfma
 }  // namespace tq

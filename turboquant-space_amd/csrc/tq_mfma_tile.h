@@ -180,13 +180,8 @@ __device__ __forceinline__ void mfma_tile(
         if constexpr (NIB) {
           decode_nib_word<!CO>(cw[c][kb], ptab, bh[c], bl[c], bt[c]);
         } else {
-          uint32_t u[8];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            u[v] = (cw[c][2 * kb] >> (8 * v)) & 0xFFu;
-            u[4 + v] = (cw[c][2 * kb + 1] >> (8 * v)) & 0xFFu;
-          }
-          decode8<REP>(u, l_tab, bh[c], bl[c], bt[c], (uint32_t)lr);
+          decode8<REP>(cw[c][2 * kb], cw[c][2 * kb + 1], l_tab, bh[c], bl[c], bt[c],
+                       (uint32_t)lr);
         }
       }
 #define TQ_PASS(ACC, A, B)                                                               \
